@@ -1,0 +1,191 @@
+"""Benchmark: synaptic events/s of the Brunel N=500k network on one MI355X.
+
+Workload (one GPU): Brunel with N = 500000 (250k Poisson at 20 Hz, 200k
+excitatory and 50k inhibitory LIF), all six projections fixed_probability
+p = 0.02 (about 2.5e9 synapses, int32 targets), weights +0.2/(N p) and
+-1.0/(N p), dt = 1e-4 s, delay 15 steps, seed {1337}.  A step is one simulation
+step (neuron updates of all populations plus delivery of the spikes due).  The
+connectivity is generated on the device before the timed region (reported as
+csr_generation_s).
+
+`value` = synaptic events delivered in the K timed steps / wall time of those
+steps (inputs resident in HBM).  `roofline` is for the delivery kernel, timed
+with HIP events on the context stream: algorithmic bytes = 4 B per synaptic
+event (int32 target) + 8 B per delivered spike (row offsets), over 8 TB/s.
+`cpu_baseline` times the CPU oracle (single core) on the same network on a
+bounded sample of steps.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode pull|push] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import subprocess
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+N, P_CONN = 500_000, 0.02
+NP, NE, NI = N // 2, N * 4 // 10, N // 10
+W_EX, W_IN = 0.2 / (N * P_CONN), -1.0 / (N * P_CONN)
+
+
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def traffic_from_profiles():
+    """HBM bytes per delivery launch from the committed rocprofv3 PMC summary of
+    this workload (profiles/deliver_traffic.json), if present."""
+    f = ROOT / "profiles" / "deliver_traffic.json"
+    if not f.exists():
+        return None
+    try:
+        return json.loads(f.read_text()).get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(net, steps_warm: int, steps_timed: int):
+    """The CPU oracle (one core) on the same network, using the device-generated
+    CSR (downloaded, checked identical in the tests), timed over a bounded sample."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib as O
+    import spice2_amd as S
+
+    L = S.runtime()
+    ctx = net.context()
+    onet = O.Net(1e-4, 15e-4, [1337])
+    P = onet.add(O.ORC_POISSON, NP, 20.0)
+    E = onet.add(O.ORC_LIF, NE)
+    I = onet.add(O.ORC_LIF, NI)
+    sizes = {P: NP, E: NE, I: NI}
+    projs = [(P, E, W_EX), (P, I, W_EX), (E, E, W_EX), (E, I, W_EX), (I, E, W_IN), (I, I, W_IN)]
+    import ctypes as C
+    for k, (s, d, w) in enumerate(projs):
+        n = C.c_int64()
+        S.check(L.spice_proj_synapses(ctx, k, C.byref(n)))
+        off = np.zeros(sizes[s] + 1, np.int64)
+        nb = np.zeros(max(n.value, 1), np.int32)
+        S.check(L.spice_proj_download(ctx, k, off.ctypes.data_as(C.c_void_p), nb.ctypes.data_as(C.c_void_p)))
+        onet.connect_csr(s, d, off, nb, 15e-4, np.float32(w))
+    for _ in range(steps_warm):
+        onet.step()
+    e0 = onet.events()
+    t0 = time.perf_counter()
+    for _ in range(steps_timed):
+        onet.step()
+    dt = time.perf_counter() - t0
+    ev = onet.events() - e0
+    onet.close()
+    return {"value": ev / dt, "unit": "synaptic events/s", "cores": 1, "kind": "port",
+            "sample": f"Brunel N=500k p=0.02, steps {steps_warm}..{steps_warm + steps_timed - 1} "
+                      f"({steps_timed} steps, {ev} events, {dt:.2f} s) on one core of "
+                      f"'{cpu_model()}' (nproc {os.cpu_count()})",
+            "ms_per_step": 1e3 * dt / steps_timed}
+
+
+def run_mode(mode: int, steps: int, warmup: int):
+    import spice2_amd as S
+
+    t0 = time.perf_counter()
+    net = S.Network.brunel(NP, NE, NI, P_CONN, 20.0, W_EX, W_IN, seeds=(1337,), mode=mode)
+    st0 = net.stats()
+    csr_s = time.perf_counter() - t0
+    net.step(warmup)
+    net.sync()
+    net.set_timing(True)
+    a = net.stats()
+    t1 = time.perf_counter()
+    net.step(steps)
+    net.sync()
+    t2 = time.perf_counter()
+    b = net.stats()
+    return net, {"events": b.events - a.events, "rows": b.rows - a.rows, "seconds": t2 - t1,
+                 "deliver_ms": b.deliver_ms - a.deliver_ms,
+                 "deliver_launches": b.deliver_launches - a.deliver_launches,
+                 "update_ms": b.update_ms - a.update_ms, "compact_ms": b.compact_ms - a.compact_ms,
+                 "csr_s": csr_s, "csr_device_ms": st0.csr_ms,
+                 "csr_host_rows": st0.csr_rows_host_resolved}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--mode", choices=["push", "pull"], default="push")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=300)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import spice2_amd as S
+    mode = S.SPICE_DELIVER_PUSH if args.mode == "push" else S.SPICE_DELIVER_PULL
+    net, r = run_mode(mode, args.steps, args.warmup)
+
+    ev = r["events"]
+    value = ev / r["seconds"]
+    # algorithmic bytes of the delivery kernel: 4 B per event (int32 target) and
+    # 8 B per delivered row (the spike's CSR offsets), counted exactly on the device
+    alg_bytes = 4.0 * ev + 8.0 * r["rows"]
+    launches = max(r["deliver_launches"], 1)
+    avg_ms = r["deliver_ms"] / launches
+    achieved = (alg_bytes / launches) / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else None
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(net, 100, args.cpu_steps)
+    net.close()
+
+    line = {
+        "metric": "synaptic events/s (Brunel N=500k, p=0.02, deterministic rasters)",
+        "value": value,
+        "unit": "synaptic events/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * r["seconds"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 state, int32 targets, u32 event counts",
+        "data": "synthetic (fixed_probability connectivity generated on the device from seed 1337)",
+        "config": {"workload": "Brunel N=500000 (P 250k / E 200k / I 50k), p=0.02, "
+                               "w=+0.2/(N p), -1/(N p), dt=1e-4, delay 15 steps, seed 1337",
+                   "delivery": args.mode, "events_timed": ev, "rows_timed": r["rows"],
+                   "csr_generation_s": r["csr_s"], "csr_device_ms": r["csr_device_ms"],
+                   "csr_rows_resolved_on_host": r["csr_host_rows"],
+                   "update_ms_per_step": r["update_ms"] / args.steps,
+                   "compact_ms_per_step": r["compact_ms"] / args.steps,
+                   "deliver_ms_per_step": r["deliver_ms"] / args.steps},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+                     "traffic": traffic_from_profiles(),
+                     "kernel": "deliver (per launch = one block of up to 15 steps)",
+                     "algorithmic_bytes_per_launch": alg_bytes / launches,
+                     "avg_launch_ms": avg_ms},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

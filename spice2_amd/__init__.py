@@ -1,0 +1,258 @@
+"""spice2_amd: MI355X-native Spice simulation engine.
+
+The product is native: libspice_hip.so (C ABI in include/spice_hip.h) and the
+C++20 user API in include/spice/.  This module only loads the in-tree native
+libraries with ctypes for tests and benchmarks; it never falls back to a
+Python or CPU implementation - a missing library raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+LIB = PKG / "lib"
+
+SPICE_OK = 0
+SPICE_EPRECONDITION = -3
+SPICE_DELIVER_PULL = 0
+SPICE_DELIVER_PUSH = 1
+SPICE_MAX_BLOCK = 16
+
+_vp = C.c_void_p
+_i64 = C.c_int64
+_u64 = C.c_uint64
+
+
+class SpiceStats(C.Structure):
+    _fields_ = [("steps", C.c_int64), ("events", C.c_uint64), ("rows", C.c_uint64),
+                ("deliver_ms", C.c_double),
+                ("deliver_launches", C.c_int64), ("update_ms", C.c_double),
+                ("update_launches", C.c_int64), ("compact_ms", C.c_double),
+                ("csr_rows_host_resolved", C.c_int64), ("csr_ambiguous_draws", C.c_int64),
+                ("csr_ms", C.c_double)]
+
+
+class SpiceError(RuntimeError):
+    pass
+
+
+class SpicePrecondition(SpiceError):
+    """A reference precondition (SPICE_PRE / SPICE_INV) failed."""
+
+
+def _require(path: Path) -> Path:
+    if not path.exists():
+        raise FileNotFoundError(
+            f"{path} is missing: build the native libraries first (python -m spice2_amd.build)")
+    return path
+
+
+_rt = None
+_wl = None
+
+
+def runtime() -> C.CDLL:
+    """libspice_hip.so with argument types set for the C ABI."""
+    global _rt
+    if _rt is None:
+        L = C.CDLL(str(_require(LIB / "libspice_hip.so")))
+        L.spice_last_error.restype = C.c_char_p
+        L.spice_ctx_create.argtypes = [C.c_int, _i64, C.POINTER(_vp)]
+        L.spice_ctx_destroy.argtypes = [_vp]
+        L.spice_ctx_set_delivery.argtypes = [_vp, C.c_int]
+        L.spice_ctx_set_timing.argtypes = [_vp, C.c_int]
+        L.spice_ctx_stream.restype = _vp
+        L.spice_ctx_stream.argtypes = [_vp]
+        L.spice_ctx_sync.argtypes = [_vp]
+        L.spice_ctx_stats.argtypes = [_vp, C.POINTER(SpiceStats)]
+        L.spice_pop_create.argtypes = [_vp, _i64, C.c_int32, C.c_int32, C.POINTER(_i64)]
+        L.spice_pop_upload.argtypes = [_vp, _i64, _vp, C.c_int32]
+        L.spice_pop_download.argtypes = [_vp, _i64, _vp, C.c_int32]
+        L.spice_pop_size.restype = _i64
+        L.spice_pop_size.argtypes = [_vp, _i64]
+        L.spice_proj_create_fixed_probability.argtypes = [_vp, _i64, _i64, C.c_double, _u64, _u64,
+                                                          _i64, C.POINTER(_i64)]
+        L.spice_proj_create_csr.argtypes = [_vp, _i64, _i64, _vp, _vp, _i64, C.POINTER(_i64)]
+        L.spice_proj_synapses.argtypes = [_vp, _i64, C.POINTER(_i64)]
+        L.spice_proj_download.argtypes = [_vp, _i64, _vp, _vp]
+        L.spice_block_end.argtypes = [_vp]
+        L.spice_spikes.argtypes = [_vp, _i64, _i64, _vp, _i64, C.POINTER(_i64)]
+        L.spice_history_len.restype = _i64
+        L.spice_history_len.argtypes = [_vp]
+        L.spice_rng_jump.argtypes = [_u64, _u64, _u64, C.POINTER(_u64), C.POINTER(_u64)]
+        L.spice_rng_jump.restype = None
+        L.spice_rng_jump_poly.argtypes = [_u64, C.POINTER(_u64), C.POINTER(_u64)]
+        L.spice_rng_jump_poly.restype = None
+        L.spice_exp_draw_interval.argtypes = [C.c_double, C.c_double, C.POINTER(C.c_double),
+                                              C.POINTER(C.c_double)]
+        L.spice_exp_draw_interval.restype = C.c_int
+        L.spice_log_cr.argtypes = [C.c_double]
+        L.spice_log_cr.restype = C.c_double
+        _rt = L
+    return _rt
+
+
+def check(status: int) -> None:
+    if status == SPICE_OK:
+        return
+    msg = runtime().spice_last_error().decode()
+    if status == SPICE_EPRECONDITION:
+        raise SpicePrecondition(msg)
+    raise SpiceError(f"spice status {status}: {msg}")
+
+
+class Context:
+    """A device context driven directly through the C ABI (no user models)."""
+
+    def __init__(self, max_delay: int = 1, device: int = 0):
+        self.L = runtime()
+        h = _vp()
+        check(self.L.spice_ctx_create(device, max_delay, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.L.spice_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def population(self, size: int, words: int = 1, draws: int = 0) -> int:
+        out = _i64()
+        check(self.L.spice_pop_create(self.h, size, words, draws, C.byref(out)))
+        return out.value
+
+    def fixed_probability(self, src: int, dst: int, p: float, seed, delay: int = 1) -> int:
+        out = _i64()
+        check(self.L.spice_proj_create_fixed_probability(self.h, src, dst, p, int(seed[0]),
+                                                         int(seed[1]), delay, C.byref(out)))
+        return out.value
+
+    def csr(self, src: int, dst: int, offsets, neighbors, delay: int = 1) -> int:
+        off = np.ascontiguousarray(offsets, np.int64)
+        nb = np.ascontiguousarray(neighbors, np.int32)
+        out = _i64()
+        check(self.L.spice_proj_create_csr(self.h, src, dst, off.ctypes.data_as(_vp),
+                                           nb.ctypes.data_as(_vp), delay, C.byref(out)))
+        return out.value
+
+    def download(self, proj: int, src_size: int):
+        n = _i64()
+        check(self.L.spice_proj_synapses(self.h, proj, C.byref(n)))
+        off = np.zeros(src_size + 1, np.int64)
+        nb = np.zeros(max(n.value, 1), np.int32)
+        check(self.L.spice_proj_download(self.h, proj, off.ctypes.data_as(_vp), nb.ctypes.data_as(_vp)))
+        return off, nb[: n.value]
+
+    def stats(self) -> SpiceStats:
+        s = SpiceStats()
+        check(self.L.spice_ctx_stats(self.h, C.byref(s)))
+        return s
+
+
+def workloads() -> C.CDLL:
+    """libspice_workloads.so: Brunel / conductance networks over the C++ API."""
+    global _wl
+    if _wl is None:
+        runtime()
+        L = C.CDLL(str(_require(LIB / "libspice_workloads.so")))
+        L.wl_error.restype = C.c_char_p
+        L.wl_brunel_create.restype = _vp
+        L.wl_brunel_create.argtypes = [_i64, _i64, _i64, C.c_double, C.c_float, C.c_float, C.c_float,
+                                       C.c_float, C.c_float, C.POINTER(C.c_uint32), C.c_int, C.c_int]
+        L.wl_coba_create.restype = _vp
+        L.wl_coba_create.argtypes = [_i64, _i64, C.c_double, C.c_float, C.c_float, C.c_float,
+                                     C.c_float, C.c_int, C.POINTER(C.c_uint32), C.c_int, C.c_int]
+        L.wl_destroy.argtypes = [_vp]
+        L.wl_step.argtypes = [_vp, _i64]
+        L.wl_sync.argtypes = [_vp]
+        L.wl_spikes.argtypes = [_vp, C.c_int, _i64, _vp, _i64, C.POINTER(_i64)]
+        L.wl_neurons.argtypes = [_vp, C.c_int, _vp, _i64]
+        L.wl_context.restype = _vp
+        L.wl_context.argtypes = [_vp]
+        _wl = L
+    return _wl
+
+
+class Network:
+    """A Brunel or conductance network built by libspice_workloads.so."""
+
+    def __init__(self, handle, sizes, state_bytes):
+        self.L = workloads()
+        if not handle:
+            msg = self.L.wl_error().decode()
+            if msg.startswith("Assertion failed"):
+                raise SpicePrecondition(msg)
+            raise SpiceError(msg)
+        self.h = handle
+        self.sizes = sizes
+        self.state_bytes = state_bytes
+
+    @classmethod
+    def brunel(cls, nP, nE, nI, p, rate, w_ex, w_in, dt=1e-4, delay=15e-4, seeds=(1337,),
+               mode=SPICE_DELIVER_PUSH):
+        L = workloads()
+        s = (C.c_uint32 * len(seeds))(*seeds)
+        h = L.wl_brunel_create(nP, nE, nI, p, rate, w_ex, w_in, dt, delay, s, len(seeds), mode)
+        return cls(h, (nP, nE, nI), 8)
+
+    @classmethod
+    def coba(cls, nE, nI, p, w_ex, w_in, dt=1e-4, delay=1e-4, init=1, seeds=(1337,),
+             mode=SPICE_DELIVER_PUSH):
+        L = workloads()
+        s = (C.c_uint32 * len(seeds))(*seeds)
+        h = L.wl_coba_create(nE, nI, p, w_ex, w_in, dt, delay, init, s, len(seeds), mode)
+        return cls(h, (nE, nI), 16)
+
+    def _check(self, r):
+        if r != 0:
+            msg = self.L.wl_error().decode()
+            raise (SpicePrecondition if r == -3 else SpiceError)(msg)
+
+    def close(self):
+        if self.h:
+            self.L.wl_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def step(self, n: int = 1):
+        self._check(self.L.wl_step(self.h, n))
+
+    def sync(self):
+        self._check(self.L.wl_sync(self.h))
+
+    def spikes(self, pop: int, age: int = 0) -> np.ndarray:
+        out = np.zeros(max(self.sizes[pop], 1), np.int32)
+        n = _i64()
+        self._check(self.L.wl_spikes(self.h, pop, age, out.ctypes.data_as(_vp), len(out), C.byref(n)))
+        return out[: n.value].copy()
+
+    def neurons(self, pop: int) -> np.ndarray:
+        """Raw AoS bytes of a stateful population, viewed as uint32 words."""
+        out = np.zeros(self.sizes[pop] * self.state_bytes // 4, np.uint32)
+        self._check(self.L.wl_neurons(self.h, pop, out.ctypes.data_as(_vp), out.nbytes))
+        return out
+
+    def context(self):
+        return self.L.wl_context(self.h)
+
+    def stats(self) -> SpiceStats:
+        s = SpiceStats()
+        check(runtime().spice_ctx_stats(self.context(), C.byref(s)))
+        return s
+
+    def set_timing(self, on: bool):
+        check(runtime().spice_ctx_set_timing(self.context(), int(on)))

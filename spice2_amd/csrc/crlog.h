@@ -91,8 +91,8 @@ SPICE_HD uint64_t to_bits(double d) {
 	return b;
 }
 
-// 1/(2i+1) for i = 0..21 as double-double (hi = RN(1/(2i+1)), lo = RN(1/(2i+1) - hi)).
-// Built at the first call from exact arithmetic on doubles: lo = -fma(hi, d, -1)/d.
+// 1/(2i+1) as double-double: hi = RN(1/d), lo = RN((1 - hi*d)/d), where the
+// residual 1 - hi*d is exact (one fma).
 SPICE_HD dd inv_odd(int i) {
 	double const d  = 2.0 * i + 1.0;
 	double const hi = 1.0 / d;
@@ -100,12 +100,9 @@ SPICE_HD dd inv_odd(int i) {
 	return {hi, r / d};
 }
 
-// Natural log, correctly rounded for positive normal and subnormal inputs.
-SPICE_HD double log_cr(double x) {
-	if (!(x > 0)) return x == 0 ? -INFINITY : NAN;
-	if (x == 1.0) return 0.0;
-	if (!(x < INFINITY)) return x;
-
+// log(x) as an unevaluated double-double (hi + lo, |lo| <= ulp(hi)/2) with a
+// relative error far below 2^-100, for positive finite x != 1.
+SPICE_HD dd log_dd(double x) {
 	uint64_t bits = to_bits(x);
 	int k         = 0;
 	if ((bits >> 52) == 0) {  // subnormal: scale into the normal range
@@ -134,8 +131,45 @@ SPICE_HD double log_cr(double x) {
 	// ln2 = 0x3FE62E42FEFA39EF + 0x3C7ABC9E3B39803F (double-double)
 	dd const ln2 = {from_bits(0x3FE62E42FEFA39EFull), from_bits(0x3C7ABC9E3B39803Full)};
 	dd const kl  = mul_d(ln2, double(k));
-	dd const r   = add(kl, lm);
+	return add(kl, lm);
+}
+
+// Natural log, correctly rounded for positive normal and subnormal inputs.
+SPICE_HD double log_cr(double x) {
+	if (!(x > 0)) return x == 0 ? -INFINITY : NAN;
+	if (x == 1.0) return 0.0;
+	if (!(x < INFINITY)) return x;
+	dd const r = log_dd(x);
 	return r.hi + r.lo;
+}
+
+// The exponential draw -scale * log(u) of the reference (random.h:270-272) for
+// u in (0, 1], as the interval [lo, hi] of values a C library log with error
+// at most `bound` ulp (glibc 2.35: 0.519, measured max 0.5183) can produce.
+// lo == hi unless log(u) lies within (bound - 0.5) ulp of a rounding midpoint
+// and the two candidate logs scale to different draws; returns lo != hi.
+SPICE_HD bool exp_draw_interval(double u, double scale, double bound, double& lo, double& hi) {
+	if (u == 1.0) {  // log(1) is exactly 0 in every C library
+		lo = hi = -scale * 0.0;
+		return false;
+	}
+	dd const L       = log_dd(u);  // negative
+	double const c   = L.hi;       // correctly rounded log(u)
+	double const e1  = -scale * c;
+	lo = hi = e1;
+	if (L.lo == 0.0) return false;
+	// neighbour of c on the side of the exact value
+	uint64_t const cb = to_bits(c);
+	double const n    = from_bits(L.lo > 0 ? cb - 1 : cb + 1);  // c < 0: toward zero is bits - 1
+	double const sp   = std::fabs(n - c);
+	double const d    = std::fabs(L.lo);
+	double const up   = std::fabs(from_bits(cb + 1) - c);  // spacing above |c|
+	double const ref  = sp > up ? sp : up;
+	if ((sp - d) > bound * ref) return false;
+	double const e2 = -scale * n;
+	lo              = e1 < e2 ? e1 : e2;
+	hi              = e1 < e2 ? e2 : e1;
+	return lo != hi;
 }
 
 }  // namespace spice_crlog

@@ -1,0 +1,610 @@
+// On-device fixed_probability generation, bit-exact with the reference
+// (topology.cpp:80-112): one xoroshiro128+ stream per projection; row `src`
+// draws e = -scale * log(u) (scale = 1/p - 1), accumulates them sequentially in
+// double (`noise`) and emits dst = index + round(noise) until dst >= dst_count or
+// index >= max_degree; every row consumes deg + 1 draws, so where a row starts
+// in the stream depends on all rows before it.
+//
+// Three phases, all on the device:
+//   A  draw:  for every block of 64 stream positions, the generator state at
+//             its start (jump-ahead) and the fixed-point sum of its draws; then
+//             an exclusive scan gives C(k), the fixed-point prefix sum of draws.
+//   B  chain: one wavefront per projection walks the rows in order.  The row
+//             starting at s ends at the first m with
+//                 (C(m) - C(s)) + (m - 1 - s) >= D - 1/2   (or at the degree cap),
+//             found with one coarse probe over the block prefixes of the next
+//             max_degree + 1 positions and one fine probe inside a block.  The
+//             fixed-point sums differ from the reference's sequential double
+//             sums by a bounded amount; a row whose end lies within that bound
+//             of the threshold is decided exactly (as in phase C).
+//   C  emit:  one wavefront per row regenerates its draws and forms the
+//             reference's sequential double sum exactly, writing the targets.
+// Exactness: each draw's log is computed in double-double.  The reference
+// calls glibc 2.35's log, which is within 0.519 ulp but not always correctly
+// rounded; a draw whose exact log lies within 0.05 ulp of a rounding midpoint
+// is carried as the interval of both candidate doubles, and the sequential sum
+// is carried as [lower, upper] trajectories (floating-point addition is
+// monotone).  A row whose targets or length differ between the trajectories -
+// it could depend on how glibc rounded - is regenerated on the host with the C
+// library's own log.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "crlog.h"
+#include "internal.h"
+
+namespace spice_rt {
+namespace {
+
+constexpr int kWave       = 64;
+constexpr int kGroup      = 4096;  // blocks of 64 positions per jump group
+constexpr double kGlibcUlp = 0.55; // bound used for glibc's log error (published 0.519)
+constexpr int kMaxUncertain = 1 << 14;
+
+struct FpParams {
+	int64_t S, D, M;
+	double scale;
+	int F;              // fractional bits of the fixed-point draw sums
+	double two_F;
+	uint64_t theta;     // (D - 1/2) * 2^F, or UINT64_MAX when unreachable
+	uint64_t tol;       // decision margin in fixed point
+	int64_t nblk;       // blocks of 64 positions covering the worst-case stream length
+};
+
+__device__ __forceinline__ double fast_draw(uint64_t x, double scale) {
+	return -scale * log(canonical_f64_leftopen(x));
+}
+__device__ __forceinline__ uint64_t quantize(double e, double two_F) {
+	return uint64_t(e * two_F + 0.5);
+}
+
+__device__ __forceinline__ xoro_state state_at(xoro_state const* block_state, int64_t pos) {
+	xoro_state st = block_state[pos >> 6];
+	for (int i = 0, n = int(pos & 63); i < n; i++) xoro_next(st);
+	return st;
+}
+
+__device__ __forceinline__ double wave_bcast(double v, int lane) {
+	return __shfl(v, lane, kWave);
+}
+
+// Phase A: block states and per-block fixed-point draw sums.
+__global__ void __launch_bounds__(256)
+fp_draw_kernel(FpParams P, xoro_state const* group_state, gf2poly const* local_poly,
+               xoro_state* block_state, uint64_t* block_sum) {
+	int64_t const stride = int64_t(gridDim.x) * blockDim.x;
+	for (int64_t b = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; b < P.nblk; b += stride) {
+		xoro_state st  = xoro_jump(group_state[b / kGroup], local_poly[b % kGroup]);
+		block_state[b] = st;
+		uint64_t sum   = 0;
+		for (int i = 0; i < 64; i++) sum += quantize(fast_draw(xoro_next(st), P.scale), P.two_F);
+		block_sum[b] = sum;
+	}
+}
+
+// Exclusive scan (mod 2^64) of n values in place; three launches.
+__global__ void __launch_bounds__(256) scan_tile_sums(uint64_t const* v, int64_t n, uint64_t* tiles) {
+	int64_t const base = int64_t(blockIdx.x) * 4096;
+	uint64_t s         = 0;
+	for (int i = threadIdx.x; i < 4096; i += 256)
+		if (base + i < n) s += v[base + i];
+	__shared__ uint64_t red[256];
+	red[threadIdx.x] = s;
+	__syncthreads();
+	for (int off = 128; off > 0; off >>= 1) {
+		if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) tiles[blockIdx.x] = red[0];
+}
+
+__global__ void __launch_bounds__(256) scan_tiles(uint64_t* tiles, int64_t ntiles, uint64_t* total) {
+	// sequential chunks per thread + block scan of chunk sums
+	int64_t const per = (ntiles + 255) / 256;
+	int64_t const a = min<int64_t>(ntiles, threadIdx.x * per), b = min<int64_t>(ntiles, a + per);
+	uint64_t s = 0;
+	for (int64_t i = a; i < b; i++) s += tiles[i];
+	__shared__ uint64_t sc[256];
+	sc[threadIdx.x] = s;
+	__syncthreads();
+	for (int off = 1; off < 256; off <<= 1) {
+		uint64_t const v = threadIdx.x >= off ? sc[threadIdx.x - off] : 0;
+		__syncthreads();
+		sc[threadIdx.x] += v;
+		__syncthreads();
+	}
+	uint64_t run = sc[threadIdx.x] - s;
+	for (int64_t i = a; i < b; i++) {
+		uint64_t const t = tiles[i];
+		tiles[i]         = run;
+		run += t;
+	}
+	if (threadIdx.x == 255) *total = sc[255];
+}
+
+__global__ void __launch_bounds__(256) scan_apply(uint64_t* v, int64_t n, uint64_t const* tiles) {
+	int64_t const base = int64_t(blockIdx.x) * 4096;
+	// each thread owns 16 consecutive values
+	int64_t const a = base + threadIdx.x * 16;
+	uint64_t loc[16];
+	uint64_t s = 0;
+	for (int i = 0; i < 16; i++) {
+		loc[i] = (a + i < n) ? v[a + i] : 0;
+		s += loc[i];
+	}
+	__shared__ uint64_t sc[256];
+	sc[threadIdx.x] = s;
+	__syncthreads();
+	for (int off = 1; off < 256; off <<= 1) {
+		uint64_t const t = threadIdx.x >= off ? sc[threadIdx.x - off] : 0;
+		__syncthreads();
+		sc[threadIdx.x] += t;
+		__syncthreads();
+	}
+	uint64_t run = tiles[blockIdx.x] + sc[threadIdx.x] - s;
+	for (int i = 0; i < 16; i++) {
+		if (a + i < n) v[a + i] = run;
+		run += loc[i];
+	}
+}
+
+// Exact row evaluation shared by phases B and C (one wavefront).  Walks the row
+// starting at stream position s with lower/upper draw intervals and sequential
+// double sums, in rounds of 1024 positions staged through LDS.
+//   len_lo / len_hi: row lengths under the lower / upper trajectory (upper
+//                    crosses first, so len_hi <= len_lo);
+//   values_certain:  targets agree between trajectories for j < min length;
+//   out:             if non-null, targets j < out_len are written (lower trajectory).
+struct RowResult {
+	int64_t len_lo, len_hi;
+	bool values_certain;
+	uint64_t ambiguous;
+};
+
+__device__ RowResult row_exact(FpParams const& P, xoro_state const* block_state, int64_t s,
+                               int32_t* out, int64_t out_len, double* lds_lo, double* lds_hi) {
+	int const lane = threadIdx.x % kWave;
+	RowResult R{-1, -1, true, 0};
+	double nlo = 0.0, nhi = 0.0;  // running sums (uniform across lanes)
+	for (int64_t j0 = 0;; j0 += 1024) {
+		// generate 16 consecutive draws per lane
+		{
+			xoro_state st = state_at(block_state, s + j0 + 16 * lane);
+			for (int t = 0; t < 16; t++) {
+				double lo, hi;
+				double const u = canonical_f64_leftopen(xoro_next(st));
+				if (spice_crlog::exp_draw_interval(u, P.scale, kGlibcUlp, lo, hi)) R.ambiguous++;
+				lds_lo[16 * lane + t] = lo;
+				lds_hi[16 * lane + t] = hi;
+			}
+		}
+		__syncthreads();
+		for (int i = 0; i < 16; i++) {
+			double const xlo = lds_lo[64 * i + lane];
+			double const xhi = lds_hi[64 * i + lane];
+			double mylo = 0.0, myhi = 0.0;
+#pragma unroll
+			for (int jj = 0; jj < kWave; jj++) {
+				nlo += wave_bcast(xlo, jj);
+				nhi += wave_bcast(xhi, jj);
+				if (lane == jj) {
+					mylo = nlo;
+					myhi = nhi;
+				}
+			}
+			int64_t const j   = j0 + 64 * i + lane;
+			int64_t const dlo = j + int64_t(round(mylo));
+			int64_t const dhi = j + int64_t(round(myhi));
+			bool const tlo    = (dlo >= P.D) || (j >= P.M);
+			bool const thi    = (dhi >= P.D) || (j >= P.M);
+			uint64_t const blo = __ballot(tlo);
+			uint64_t const bhi = __ballot(thi);
+			if (R.len_hi < 0 && bhi) R.len_hi = j0 + 64 * i + (__ffsll((unsigned long long)bhi) - 1);
+			int64_t const first_lo =
+			    blo ? j0 + 64 * i + (__ffsll((unsigned long long)blo) - 1) : INT64_MAX;
+			int64_t const lim = min(first_lo, R.len_hi >= 0 ? R.len_hi : INT64_MAX);
+			if (j < lim && dlo != dhi) R.values_certain = false;
+			if (out && j < out_len && j < first_lo) out[j] = int32_t(dlo);
+			if (blo) {
+				R.len_lo = first_lo;
+				break;
+			}
+		}
+		__syncthreads();
+		if (R.len_lo >= 0) break;
+	}
+	// reduce per-lane ambiguity counts and certainty
+	for (int off = 32; off > 0; off >>= 1) R.ambiguous += __shfl_xor(R.ambiguous, off, kWave);
+	R.values_certain = __all(R.values_certain);
+	return R;
+}
+
+// Phase B: walk the row chain.  One 64-thread workgroup per projection.
+struct ChainJob {
+	FpParams P;
+	xoro_state const* block_state;
+	uint64_t const* block_prefix;  // nblk + 1 entries
+	int64_t* row_start;            // S + 1 entries
+	int64_t r0, s0;                // resume point
+	int32_t status;                // 0 done, 1 host needed at row `status_row`, 2 error
+	int64_t status_row;
+	unsigned long long ambiguous;
+	int64_t exact_rows;
+};
+
+__device__ uint64_t prefix_at(FpParams const& P, xoro_state const* block_state,
+                              uint64_t const* block_prefix, int64_t pos) {
+	// C(pos): block prefix plus the draws of pos's block before pos (wave-parallel)
+	int const lane     = threadIdx.x % kWave;
+	int64_t const b    = pos >> 6;
+	int const r        = int(pos & 63);
+	uint64_t q         = 0;
+	if (lane < r) {
+		xoro_state st = state_at(block_state, (b << 6) + lane);
+		q             = quantize(fast_draw(xoro_next(st), P.scale), P.two_F);
+	}
+	for (int off = 32; off > 0; off >>= 1) q += __shfl_xor(q, off, kWave);
+	return block_prefix[b] + q;
+}
+
+__global__ void __launch_bounds__(64) fp_chain_kernel(ChainJob* jobs) {
+	__shared__ double lds_lo[1024], lds_hi[1024];
+	ChainJob& J      = jobs[blockIdx.x];
+	FpParams const P = J.P;
+	int const lane   = threadIdx.x;
+	int64_t s        = J.s0;
+	uint64_t Cs      = prefix_at(P, J.block_state, J.block_prefix, s);
+	unsigned long long amb = 0;
+	int64_t exact = 0;
+	for (int64_t r = J.r0; r < P.S; r++) {
+		if (lane == 0) J.row_start[r] = s;
+		int64_t const last = s + P.M + 1;  // cap position for m
+		// coarse: block boundaries q in [s+1, last]
+		int64_t const b_lo = (s >> 6) + 1, b_hi = last >> 6;
+		int64_t fb = -1;  // block holding m - 1
+		for (int64_t b0 = b_lo; b0 <= b_hi && fb < 0; b0 += kWave) {
+			int64_t const b = b0 + lane;
+			bool hit        = false;
+			if (b <= b_hi) {
+				uint64_t const G = (J.block_prefix[b] - Cs) + (uint64_t((b << 6) - 1 - s) << P.F);
+				hit              = G >= P.theta;
+			}
+			uint64_t const m = __ballot(hit);
+			if (m) fb = b0 + (__ffsll((unsigned long long)m) - 1) - 1;
+		}
+		if (fb < 0) fb = (last - 1) >> 6;  // no crossing before the cap: block of position last-1
+		// fine: positions P_j = 64 fb + j, G at P_j + 1
+		int64_t const pj = (fb << 6) + lane;
+		uint64_t qv      = 0;
+		{
+			xoro_state st = state_at(J.block_state, pj);
+			qv            = quantize(fast_draw(xoro_next(st), P.scale), P.two_F);
+		}
+		uint64_t incl = qv;  // inclusive wave scan
+		for (int off = 1; off < kWave; off <<= 1) {
+			uint64_t const t = __shfl_up(incl, off, kWave);
+			if (lane >= off) incl += t;
+		}
+		uint64_t const Cnext = J.block_prefix[fb] + incl;  // C(pj + 1)
+		bool const valid     = pj >= s && pj <= last - 1;
+		uint64_t const G     = (Cnext - Cs) + (uint64_t(pj - s) << P.F);
+		uint64_t const cross = __ballot(valid && G >= P.theta);
+		int64_t m;
+		int jl;  // lane of position m - 1
+		if (cross) {
+			jl = __ffsll((unsigned long long)cross) - 1;
+			m  = (fb << 6) + jl + 1;
+		} else {
+			m  = last;
+			jl = int(last - 1 - (fb << 6));
+		}
+		// margin checks on the decisions that fix the row length
+		bool uncertain = false;
+		uint64_t const G_m  = __shfl(G, jl, kWave);
+		uint64_t const G_pm = jl > 0 ? __shfl(G, jl - 1, kWave) : 0;
+		if (cross && m < last && G_m - P.theta < P.tol) uncertain = true;
+		if (m - 1 > s) {  // position m-2 did not cross
+			uint64_t gp;
+			if (jl > 0)
+				gp = G_pm;
+			else  // m - 1 is the first position of block fb: boundary value
+				gp = (J.block_prefix[fb] - Cs) + (uint64_t((fb << 6) - 1 - s) << P.F);
+			if (!(gp < P.theta && P.theta - gp >= P.tol)) uncertain = true;
+		}
+		uint64_t Cm = __shfl(Cnext, jl, kWave);
+		if (uncertain) {
+			RowResult const R = row_exact(P, J.block_state, s, nullptr, 0, lds_lo, lds_hi);
+			amb += R.ambiguous;
+			exact++;
+			if (R.len_lo != R.len_hi) {
+				if (lane == 0) {
+					J.status     = 1;
+					J.status_row = r;
+				}
+				return;
+			}
+			m  = s + R.len_lo + 1;
+			Cm = prefix_at(P, J.block_state, J.block_prefix, m);
+		}
+		s  = m;
+		Cs = Cm;
+	}
+	if (lane == 0) {
+		J.row_start[P.S] = s;
+		J.status         = 0;
+		J.ambiguous += amb;
+		J.exact_rows += exact;
+	}
+}
+
+// Phase C: emit every row.  One 64-thread workgroup per row (grid-stride).
+__global__ void __launch_bounds__(64)
+fp_emit_kernel(FpParams P, xoro_state const* block_state, int64_t const* row_start,
+               int64_t* offsets, int32_t* neighbors, int64_t* uncertain, int32_t* n_uncertain,
+               int32_t* error, unsigned long long* ambiguous) {
+	__shared__ double lds_lo[1024], lds_hi[1024];
+	unsigned long long amb = 0;
+	for (int64_t r = blockIdx.x; r < P.S; r += gridDim.x) {
+		int64_t const s   = row_start[r];
+		int64_t const len = row_start[r + 1] - s - 1;
+		int64_t const off = s - r;
+		if (threadIdx.x == 0) offsets[r] = off;
+		RowResult const R = row_exact(P, block_state, s, neighbors + off, len, lds_lo, lds_hi);
+		amb += R.ambiguous;
+		if (R.len_lo != len || R.len_hi != len || !R.values_certain) {
+			if (threadIdx.x == 0) {
+				if (R.len_lo == R.len_hi && R.len_lo != len) {
+					atomicExch(error, 3);  // chain length disagrees with an exact row
+				} else {
+					int const k = atomicAdd(n_uncertain, 1);
+					if (k < kMaxUncertain)
+						uncertain[k] = r;
+					else
+						atomicExch(error, 4);
+				}
+			}
+		}
+	}
+	if (threadIdx.x == 0) {
+		if (blockIdx.x == 0) offsets[P.S] = row_start[P.S] - P.S;
+		atomicAdd(ambiguous, amb);
+	}
+}
+
+// ------------------------------------------------------------------- host
+
+// The reference row computation with the C library's log (topology.cpp:93-110);
+// used for rows the device proved could depend on glibc's rounding.
+int64_t host_row(xoro_state st, FpParams const& P, std::vector<int32_t>& out) {
+	out.clear();
+	int32_t index = 0;
+	double noise  = 0;
+	for (;;) {
+		double const u = canonical_f64_leftopen(xoro_next(st));
+		noise += -P.scale * std::log(u);
+		int32_t const dst = index + static_cast<int32_t>(std::round(noise));
+		if ((dst >= P.D) | (index >= P.M)) break;
+		out.push_back(dst);
+		index++;
+	}
+	return index;
+}
+
+std::vector<gf2poly> const& local_polys() {
+	static std::vector<gf2poly> v;
+	static std::once_flag once;
+	std::call_once(once, [] {
+		v.resize(kGroup);
+		gf2poly const step = xoro_jump_poly(64);
+		gf2poly cur{1, 0};
+		for (int j = 0; j < kGroup; j++) {
+			v[j] = cur;
+			cur  = gf2_mulmod(cur, step);
+		}
+	});
+	return v;
+}
+
+double ulp_of(double x) {
+	x = std::fabs(x);
+	return std::nextafter(x, INFINITY) - x;
+}
+
+template <class T>
+struct DevBuf {
+	T* p = nullptr;
+	~DevBuf() {
+		if (p) (void)hipFree(p);
+	}
+	hipError_t alloc(size_t n) { return hipMalloc(reinterpret_cast<void**>(&p), sizeof(T) * std::max<size_t>(n, 1)); }
+};
+
+}  // namespace
+
+int generate_fixed_probability(spice_ctx* ctx, Proj& pr, int64_t S, int64_t D, double p,
+                               xoro_state seed) {
+	auto const t_begin = std::chrono::steady_clock::now();
+	hipStream_t const st = ctx->stream;
+	// max_degree exactly as topology.cpp:74-75 (double arithmetic, truncation)
+	int64_t const M = int64_t(double(D) * p + 3 * std::sqrt(double(D) * p * (1 - p)));
+	SPICE_HIP(hipMalloc(reinterpret_cast<void**>(&pr.offsets), sizeof(int64_t) * (S + 1)));
+	if (D == 0 || p == 0) {  // topology.cpp:84-85: all rows empty
+		SPICE_HIP(hipMemsetAsync(pr.offsets, 0, sizeof(int64_t) * (S + 1), st));
+		SPICE_HIP(hipMalloc(reinterpret_cast<void**>(&pr.neighbors), sizeof(int32_t)));
+		pr.nnz = 0;
+		return SPICE_OK;
+	}
+	FpParams P{};
+	P.S     = S;
+	P.D     = D;
+	P.M     = M;
+	P.scale = 1 / p - 1;
+	// fixed point: differences over M + 1 positions stay below 2^62
+	double const emax  = P.scale * 37.5 + 1.0;  // -log(2^-53) = 36.74
+	double const span  = double(M + 2) * emax;
+	P.F                = std::min(52, int(std::floor(62.0 - std::log2(span))));
+	if (P.F < 8)
+		return set_error(SPICE_EINVAL, "fixed_probability: connection probability too small for "
+		                               "this target population size");
+	P.two_F            = std::ldexp(1.0, P.F);
+	double const th    = (double(D) - 0.5) * P.two_F;  // exact: F >= 1
+	// every G value is below span * 2^F < 2^62, so a threshold at or above 2^62 is never reached
+	P.theta            = th >= std::ldexp(1.0, 62) ? UINT64_MAX : uint64_t(th);
+	// decision margin: sequential double rounding + log error + quantisation, x2
+	double const nmax  = double(D) + emax;
+	double const tol   = 2.0 * double(M + 2) *
+	                   (0.5 * ulp_of(nmax) + 8.0 * ulp_of(emax) + std::ldexp(1.0, -P.F)) + 1e-12;
+	P.tol              = uint64_t(std::ceil(tol * P.two_F)) + 2;
+	// SPICE_CSR_EXACT_ROWS=1 decides every row's length with the exact interval
+	// evaluation instead of the fixed-point probe (used to test that path).
+	if (char const* e = std::getenv("SPICE_CSR_EXACT_ROWS"); e && e[0] == '1') P.tol = UINT64_MAX / 2;
+	int64_t const L    = S * (M + 1);  // worst-case stream length
+	// every kernel reads positions below L + 1024 (a row evaluation may run one
+	// 1024-position round past the last row's end)
+	P.nblk             = (L + 1024 + 63) / 64 + 1;
+
+	// phase A
+	int64_t const ngroups = (P.nblk + kGroup - 1) / kGroup;
+	std::vector<xoro_state> groups(ngroups);
+	{
+		gf2poly const gstep = xoro_jump_poly(uint64_t(64) * kGroup);
+		xoro_state g        = seed;
+		for (int64_t i = 0; i < ngroups; i++) {
+			groups[i] = g;
+			g         = xoro_jump(g, gstep);
+		}
+	}
+	DevBuf<xoro_state> d_groups, d_block_state;
+	DevBuf<gf2poly> d_local;
+	DevBuf<uint64_t> d_prefix, d_tiles, d_total;
+	SPICE_HIP(d_groups.alloc(ngroups));
+	SPICE_HIP(d_local.alloc(kGroup));
+	SPICE_HIP(d_block_state.alloc(P.nblk));
+	SPICE_HIP(d_prefix.alloc(P.nblk + 1));
+	int64_t const ntiles = (P.nblk + 1 + 4095) / 4096;
+	SPICE_HIP(d_tiles.alloc(ntiles));
+	SPICE_HIP(d_total.alloc(1));
+	SPICE_HIP(hipMemcpyAsync(d_groups.p, groups.data(), sizeof(xoro_state) * ngroups,
+	                         hipMemcpyHostToDevice, st));
+	SPICE_HIP(hipMemcpyAsync(d_local.p, local_polys().data(), sizeof(gf2poly) * kGroup,
+	                         hipMemcpyHostToDevice, st));
+	SPICE_HIP(hipMemsetAsync(d_prefix.p, 0, sizeof(uint64_t) * (P.nblk + 1), st));
+	{
+		int64_t const grid = std::min<int64_t>((P.nblk + 255) / 256, 65536);
+		hipLaunchKernelGGL(fp_draw_kernel, dim3(unsigned(grid)), dim3(256), 0, st, P, d_groups.p,
+		                   d_local.p, d_block_state.p, d_prefix.p);
+		SPICE_HIP(hipGetLastError());
+	}
+	int64_t const n = P.nblk + 1;  // includes a trailing zero: its prefix is the total
+	hipLaunchKernelGGL(scan_tile_sums, dim3(unsigned(ntiles)), dim3(256), 0, st, d_prefix.p, n, d_tiles.p);
+	hipLaunchKernelGGL(scan_tiles, dim3(1), dim3(256), 0, st, d_tiles.p, ntiles, d_total.p);
+	hipLaunchKernelGGL(scan_apply, dim3(unsigned(ntiles)), dim3(256), 0, st, d_prefix.p, n, d_tiles.p);
+	SPICE_HIP(hipGetLastError());
+
+	// phase B
+	DevBuf<int64_t> d_row_start;
+	DevBuf<ChainJob> d_job;
+	SPICE_HIP(d_row_start.alloc(S + 1));
+	SPICE_HIP(d_job.alloc(1));
+	ChainJob job{};
+	job.P            = P;
+	job.block_state  = d_block_state.p;
+	job.block_prefix = d_prefix.p;
+	job.row_start    = d_row_start.p;
+	job.r0           = 0;
+	job.s0           = 0;
+	int64_t host_rows = 0;
+	std::vector<int32_t> row;
+	for (;;) {
+		SPICE_HIP(hipMemcpyAsync(d_job.p, &job, sizeof(job), hipMemcpyHostToDevice, st));
+		hipLaunchKernelGGL(fp_chain_kernel, dim3(1), dim3(64), 0, st, d_job.p);
+		SPICE_HIP(hipGetLastError());
+		SPICE_HIP(hipMemcpyAsync(&job, d_job.p, sizeof(job), hipMemcpyDeviceToHost, st));
+		SPICE_HIP(hipStreamSynchronize(st));
+		if (job.status == 0) break;
+		if (job.status != 1) return set_error(SPICE_EDEVICE, "fixed_probability chain walk failed");
+		// the row's length depends on glibc's rounding: settle it with the C library
+		int64_t const r = job.status_row;
+		int64_t s       = 0;
+		SPICE_HIP(hipMemcpy(&s, d_row_start.p + r, sizeof(s), hipMemcpyDeviceToHost));
+		xoro_state bs;
+		SPICE_HIP(hipMemcpy(&bs, d_block_state.p + (s >> 6), sizeof(bs), hipMemcpyDeviceToHost));
+		for (int i = 0; i < int(s & 63); i++) xoro_next(bs);
+		int64_t const len = host_row(bs, P, row);
+		host_rows++;
+		job.r0     = r + 1;
+		job.s0     = s + len + 1;
+		job.status = 0;
+		if (r + 1 == S) {  // last row: record the end and finish
+			int64_t const end = job.s0;
+			SPICE_HIP(hipMemcpy(d_row_start.p + S, &end, sizeof(end), hipMemcpyHostToDevice));
+			break;
+		}
+	}
+	int64_t end = 0;
+	SPICE_HIP(hipMemcpy(&end, d_row_start.p + S, sizeof(end), hipMemcpyDeviceToHost));
+	pr.nnz = end - S;
+	if (pr.nnz < 0 || end > L) return set_error(SPICE_EDEVICE, "fixed_probability chain out of range");
+
+	// phase C
+	SPICE_HIP(hipMalloc(reinterpret_cast<void**>(&pr.neighbors), sizeof(int32_t) * std::max<int64_t>(pr.nnz, 1)));
+	DevBuf<int64_t> d_unc;
+	DevBuf<int32_t> d_nunc;
+	DevBuf<unsigned long long> d_amb;
+	SPICE_HIP(d_unc.alloc(kMaxUncertain));
+	SPICE_HIP(d_nunc.alloc(1));
+	SPICE_HIP(d_amb.alloc(1));
+	SPICE_HIP(hipMemsetAsync(d_nunc.p, 0, sizeof(int32_t), st));
+	SPICE_HIP(hipMemsetAsync(d_amb.p, 0, sizeof(unsigned long long), st));
+	SPICE_HIP(hipMemsetAsync(ctx->error, 0, sizeof(int32_t), st));
+	{
+		int64_t const grid = std::min<int64_t>(S, 65536);
+		hipLaunchKernelGGL(fp_emit_kernel, dim3(unsigned(grid)), dim3(64), 0, st, P, d_block_state.p,
+		                   d_row_start.p, pr.offsets, pr.neighbors, d_unc.p, d_nunc.p, ctx->error,
+		                   d_amb.p);
+		SPICE_HIP(hipGetLastError());
+	}
+	int32_t nunc = 0, err = 0;
+	unsigned long long amb = 0;
+	SPICE_HIP(hipMemcpyAsync(&nunc, d_nunc.p, sizeof(nunc), hipMemcpyDeviceToHost, st));
+	SPICE_HIP(hipMemcpyAsync(&err, ctx->error, sizeof(err), hipMemcpyDeviceToHost, st));
+	SPICE_HIP(hipMemcpyAsync(&amb, d_amb.p, sizeof(amb), hipMemcpyDeviceToHost, st));
+	SPICE_HIP(hipStreamSynchronize(st));
+	if (err) {
+		(void)hipMemset(ctx->error, 0, sizeof(int32_t));
+		return set_error(SPICE_EDEVICE, err == 3 ? "fixed_probability: row length disagreement"
+		                                         : "fixed_probability: too many undecided rows");
+	}
+	if (nunc > 0) {
+		std::vector<int64_t> rows(nunc), starts(2);
+		SPICE_HIP(hipMemcpy(rows.data(), d_unc.p, sizeof(int64_t) * nunc, hipMemcpyDeviceToHost));
+		for (int64_t r : rows) {
+			SPICE_HIP(hipMemcpy(starts.data(), d_row_start.p + r, sizeof(int64_t) * 2, hipMemcpyDeviceToHost));
+			xoro_state bs;
+			SPICE_HIP(hipMemcpy(&bs, d_block_state.p + (starts[0] >> 6), sizeof(bs), hipMemcpyDeviceToHost));
+			for (int i = 0; i < int(starts[0] & 63); i++) xoro_next(bs);
+			int64_t const len = host_row(bs, P, row);
+			if (len != starts[1] - starts[0] - 1)
+				return set_error(SPICE_EDEVICE, "fixed_probability: host row length disagreement");
+			if (len)
+				SPICE_HIP(hipMemcpy(pr.neighbors + (starts[0] - r), row.data(), sizeof(int32_t) * len,
+				                    hipMemcpyHostToDevice));
+			host_rows++;
+		}
+	}
+	ctx->stats.csr_rows_host_resolved += host_rows;
+	ctx->stats.csr_ambiguous_draws += int64_t(amb + job.ambiguous);
+	ctx->stats.csr_ms +=
+	    std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_begin).count();
+	return SPICE_OK;
+}
+
+}  // namespace spice_rt

@@ -1,0 +1,226 @@
+// Benchmark and test networks written against the user API (include/spice/snn.h),
+// exported with a C ABI for the Python tests and bench.py.
+//
+// The model structs are the reference samples' workloads (samples/brunel.cpp:23-70,
+// samples/vogels.cpp:10-59) with their floating-point expressions unchanged;
+// the Poisson rate is a member so that swept rates reuse the same struct (at
+// 20 Hz it computes exactly what the sample's constant does).
+#include <cstring>
+#include <exception>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "spice/snn.h"
+
+using namespace spice;
+using namespace spice::util;
+
+namespace wl {
+struct poisson {
+	float rate = 20;
+	bool update(float dt, auto& rng) const { return util::generate_canonical<float>(rng) < (rate * dt); }
+};
+
+struct lif {
+	struct neuron {
+		float V   = 0;
+		int Twait = 0;
+	};
+	bool update(neuron& n, float dt, auto) const {
+		float const TmemInv = 1.0 / 0.02;
+		float const Vrest   = 0.0;
+		int const Tref      = 20;
+		float const Vthres  = 0.02;
+		if (--n.Twait <= 0) {
+			if (n.V > Vthres) {
+				n.V     = Vrest;
+				n.Twait = Tref;
+				return true;
+			}
+			n.V += (Vrest - n.V) * (dt * TmemInv);
+		}
+		return false;
+	}
+};
+
+struct fixed_weight {
+	float weight;
+	void deliver(lif::neuron& to) const { to.V += weight; }
+};
+
+struct coba {
+	int init_mode = 0;  // 1: V ~ U[-0.06, -0.05)
+	struct neuron {
+		float V     = -0.06;
+		float Gex   = 0;
+		float Gin   = 0;
+		Int32 Twait = 0;
+	};
+	void init(neuron& n, Int, auto& rng) const {
+		if (init_mode == 1) n.V = uniform_real_distribution<float>(-0.06f, -0.05f)(rng);
+	}
+	bool update(neuron& n, float const dt, auto&) const {
+		Int32 const Tref    = 50;
+		float const Vrest   = -0.06;
+		float const Vthres  = -0.05;
+		float const TmemInv = 1.0f / 0.02;
+		float const Eex     = 0.0;
+		float const Ein     = -0.08;
+		float const Ibg     = 0.02;
+		float const TexInv  = 1.0f / 0.005;
+		float const TinInv  = 1.0f / 0.01;
+		bool spiked         = false;
+		if (--n.Twait <= 0) {
+			if (n.V > Vthres) {
+				n.V     = Vrest;
+				n.Twait = Tref;
+				spiked  = true;
+			} else
+				n.V += ((Vrest - n.V) + n.Gex * (Eex - n.V) + n.Gin * (Ein - n.V) + Ibg) * (dt * TmemInv);
+		}
+		n.Gex -= n.Gex * (dt * TexInv);
+		n.Gin -= n.Gin * (dt * TinInv);
+		return spiked;
+	}
+};
+struct excitatory {
+	float weight;
+	void deliver(coba::neuron& to) const { to.Gex += weight; }
+};
+struct inhibitory {
+	float weight;
+	void deliver(coba::neuron& to) const { to.Gin += weight; }
+};
+}  // namespace wl
+
+#include "spice/host_code.h"
+
+namespace {
+thread_local std::string g_err;
+
+struct network {
+	std::unique_ptr<snn> net;
+	std::vector<spice::detail::NeuronPopulation*> pops;
+	int kind = 0;  // 0 Brunel (P, E, I lif), 1 conductance (E, I)
+};
+
+int fail(std::exception const& e) {
+	g_err = e.what();
+	return dynamic_cast<std::logic_error const*>(&e) ? -3 : -1;
+}
+
+seed_seq make_seed(uint32_t const* seeds, int n) {
+	if (n <= 0) throw std::logic_error("Assertion failed (random.h:151): il.size() > 0");
+	UInt128 const h = util::detail::murmur3(seeds, 4 * UInt(n));
+	return seed_seq::from_value(h);
+}
+}  // namespace
+
+extern "C" {
+
+char const* wl_error() { return g_err.c_str(); }
+
+// Brunel network of samples/brunel.cpp:75-103 with parameterised sizes,
+// probability, Poisson rate, weights and seed.  mode: SPICE_DELIVER_*.
+void* wl_brunel_create(int64_t nP, int64_t nE, int64_t nI, double p, float rate, float wex, float win,
+                       float dt, float delay, uint32_t const* seeds, int nseeds, int mode) {
+	try {
+		auto w  = std::make_unique<network>();
+		w->net  = std::make_unique<snn>(dt, delay, make_seed(seeds, nseeds));
+		auto& s = *w->net;
+		s.set_delivery(mode);
+		auto P = s.add_population<wl::poisson>(nP, {rate});
+		auto E = s.add_population<wl::lif>(nE);
+		auto I = s.add_population<wl::lif>(nI);
+		s.connect<wl::fixed_weight>(P, E, fixed_probability(p), delay, {wex});
+		s.connect<wl::fixed_weight>(P, I, fixed_probability(p), delay, {wex});
+		s.connect<wl::fixed_weight>(E, E, fixed_probability(p), delay, {wex});
+		s.connect<wl::fixed_weight>(E, I, fixed_probability(p), delay, {wex});
+		s.connect<wl::fixed_weight>(I, E, fixed_probability(p), delay, {win});
+		s.connect<wl::fixed_weight>(I, I, fixed_probability(p), delay, {win});
+		w->pops = {P, E, I};
+		w->kind = 0;
+		return w.release();
+	} catch (std::exception const& e) {
+		fail(e);
+		return nullptr;
+	}
+}
+
+// Conductance-based E/I network (samples/vogels.cpp:10-73), init_mode 1 draws V0.
+void* wl_coba_create(int64_t nE, int64_t nI, double p, float wex, float win, float dt, float delay,
+                     int init_mode, uint32_t const* seeds, int nseeds, int mode) {
+	try {
+		auto w  = std::make_unique<network>();
+		w->net  = std::make_unique<snn>(dt, delay, make_seed(seeds, nseeds));
+		auto& s = *w->net;
+		s.set_delivery(mode);
+		auto E = s.add_population<wl::coba>(nE, {init_mode});
+		auto I = s.add_population<wl::coba>(nI, {init_mode});
+		s.connect<wl::excitatory>(E, E, fixed_probability(p), delay, {wex});
+		s.connect<wl::excitatory>(E, I, fixed_probability(p), delay, {wex});
+		s.connect<wl::inhibitory>(I, E, fixed_probability(p), delay, {win});
+		s.connect<wl::inhibitory>(I, I, fixed_probability(p), delay, {win});
+		w->pops = {E, I};
+		w->kind = 1;
+		return w.release();
+	} catch (std::exception const& e) {
+		fail(e);
+		return nullptr;
+	}
+}
+
+void wl_destroy(void* h) { delete static_cast<network*>(h); }
+
+int wl_step(void* h, int64_t n) {
+	try {
+		auto* w = static_cast<network*>(h);
+		for (int64_t i = 0; i < n; i++) w->net->step();
+		return 0;
+	} catch (std::exception const& e) {
+		return fail(e);
+	}
+}
+
+int wl_sync(void* h) {
+	try {
+		static_cast<network*>(h)->net->sync();
+		return 0;
+	} catch (std::exception const& e) {
+		return fail(e);
+	}
+}
+
+int wl_spikes(void* h, int pop, int64_t age, int32_t* out, int64_t cap, int64_t* count) {
+	try {
+		auto* w = static_cast<network*>(h);
+		auto s  = w->pops.at(size_t(pop))->spikes(age);
+		*count  = int64_t(s.size());
+		if (int64_t(s.size()) > cap) throw std::runtime_error("spike buffer too small");
+		std::memcpy(out, s.data(), sizeof(int32_t) * s.size());
+		return 0;
+	} catch (std::exception const& e) {
+		return fail(e);
+	}
+}
+
+// Copies a stateful population's neurons (the reference's AoS layout).
+int wl_neurons(void* h, int pop, void* out, int64_t bytes) {
+	try {
+		auto* w = static_cast<network*>(h);
+		auto* p = w->pops.at(size_t(pop));
+		void* src = p->neurons();
+		if (!src) throw std::runtime_error("stateless population");
+		int64_t const need = p->size() * (w->kind == 0 ? int64_t(sizeof(wl::lif::neuron))
+		                                               : int64_t(sizeof(wl::coba::neuron)));
+		if (bytes < need) throw std::runtime_error("state buffer too small");
+		std::memcpy(out, src, size_t(need));
+		return 0;
+	} catch (std::exception const& e) {
+		return fail(e);
+	}
+}
+
+spice_ctx* wl_context(void* h) { return static_cast<network*>(h)->net->context(); }
+}
