@@ -129,7 +129,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--mode", choices=["push", "pull"], default="push")
+    ap.add_argument("--mode", choices=["push", "pull"], default="pull")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=300)
     args = ap.parse_args()

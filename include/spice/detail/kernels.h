@@ -123,7 +123,7 @@ __device__ inline void apply_deliveries(N& n, spice_pop_view const& v, replay_sl
 		uint32_t* cp     = v.in[c].counts + (step % v.in[c].slots) * v.size + i;
 		uint32_t const k = *cp;
 		if (k == 0) continue;
-		*cp = 0;
+		if (v.clear_counts) *cp = 0;
 		replay_slot const s = slots[c];
 		if constexpr (!std::is_void_v<Syn>) {
 			if (s.fn == nullptr) {
@@ -172,6 +172,22 @@ update_lanes_kernel(spice_pop_view v, spice_block_view b, Neur neur, replay_slot
 	}
 	if constexpr (!std::is_void_v<N>)
 		if (active) store_state(n, v.state, v.size, i);
+}
+
+// Applies the deliveries counted for `step` (the next step to run) to the
+// stored state, so that host reads see the state the reference has after the
+// previous step; the counters are cleared, so the next update does not apply
+// them again.
+template <class Neur, class Syn>
+__global__ void __launch_bounds__(256)
+apply_pending_kernel(spice_pop_view v, int64_t step, replay_slot const* slots) {
+	using N         = typename neuron_traits<Neur>::type;
+	int64_t const i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (i >= v.size) return;
+	N n;
+	load_state(n, v.state, v.size, i);
+	apply_deliveries<Syn>(n, v, slots, step, i);
+	store_state(n, v.state, v.size, i);
 }
 
 // One thread per chunk of 16 consecutive neurons (neurons take `draws` draws

@@ -143,6 +143,7 @@ public:
 		static_assert(stateful, "Can only return collections of stateful neurons.");
 		_eng->flush();
 		if (!_host_valid) {
+			apply_pending();
 			check_spice(spice_pop_download(_eng->ctx, _id, _host.data(), int32_t(sizeof(neuron_t))));
 			_host_valid = true;
 		}
@@ -185,6 +186,14 @@ public:
 			_launch_primary = [this](spice_pop_view const& v, spice_block_view const& b, hipStream_t s) {
 				run<Syn>(v, b, s);
 			};
+			_apply_primary = [this](spice_pop_view const& v, int64_t step, hipStream_t s) {
+				if constexpr (stateful) {
+					unsigned const grid = unsigned((_size + 255) / 256);
+					hipLaunchKernelGGL((apply_pending_kernel<Neur, Syn>), dim3(grid), dim3(256), 0, s, v, step,
+					                   _slots_dev);
+					check_hip(hipGetLastError(), "apply pending deliveries");
+				}
+			};
 		}
 		_slots_dirty = true;
 	}
@@ -218,6 +227,18 @@ public:
 			run<void>(v, b, s);
 	}
 
+	// Applies the deliveries already counted for the next step (see kernels.h).
+	void apply_pending() {
+		if constexpr (stateful) {
+			if (_in.empty() || _size == 0) return;
+			prepare_block();
+			spice_pop_view v;
+			check_spice(spice_pop_view_now(_eng->ctx, _id, &v));
+			int64_t const step = spice_steps(_eng->ctx);
+			_apply_primary(v, step, _eng->stream());
+		}
+	}
+
 private:
 	engine* _eng;
 	Neur _neuron;
@@ -231,6 +252,7 @@ private:
 	std::vector<std::unique_ptr<void, void (*)(void*)>> _syn_objs;
 	std::type_index _primary = std::type_index(typeid(void));
 	std::function<void(spice_pop_view const&, spice_block_view const&, hipStream_t)> _launch_primary;
+	std::function<void(spice_pop_view const&, int64_t, hipStream_t)> _apply_primary;
 	replay_slot* _slots_dev = nullptr;
 	bool _slots_dirty       = true;
 

@@ -68,6 +68,7 @@ typedef struct {
 	spice_in_view in[SPICE_MAX_IN]; /* connect order */
 	spice_rng_state rng[SPICE_MAX_BLOCK]; /* per step: stream state at this population's first draw */
 	int32_t* error;         /* device flag: nonzero after a failed device-side check */
+	int32_t clear_counts;   /* the update kernel clears the counters it consumes (push mode) */
 } spice_pop_view;
 
 typedef struct {
@@ -97,7 +98,12 @@ char const* spice_last_error(void);
  * projection's delay and the spike history kept for spice_spikes(). */
 int spice_ctx_create(int device, int64_t max_delay, spice_ctx** out);
 void spice_ctx_destroy(spice_ctx* ctx);
+/* Delivery algorithm (default SPICE_DELIVER_PULL); fixed once steps have run. */
 int spice_ctx_set_delivery(spice_ctx* ctx, int mode);
+/* Targets per pull tile (multiple of 64, <= 65536); the runtime lowers it so that
+ * the counters of a tile's incoming projections fit 64 KB of LDS. */
+int spice_ctx_set_tile(spice_ctx* ctx, int64_t tile);
+int64_t spice_ctx_tile(spice_ctx* ctx);
 int spice_ctx_set_timing(spice_ctx* ctx, int enabled);
 void* spice_ctx_stream(spice_ctx* ctx); /* hipStream_t all work of the context runs on */
 int spice_ctx_sync(spice_ctx* ctx);     /* waits and reports device-side errors */
@@ -147,6 +153,13 @@ int spice_proj_download(spice_ctx* ctx, int64_t proj, int64_t* offsets, int32_t*
 int spice_block_begin(spice_ctx* ctx, int32_t nsteps, float const* dt, uint64_t const* seeds,
                       spice_block_view* block, spice_pop_view* pops, int32_t max_pops);
 int spice_block_end(spice_ctx* ctx);
+/* The view of a population outside a block (no RNG fields): used to apply the
+ * deliveries already counted for the next step before the host reads neuron
+ * state, since the reference applies each step's deliveries at the end of that
+ * step (snn.cpp:21-25) while the device applies them at the start of the next. */
+int spice_pop_view_now(spice_ctx* ctx, int64_t pop, spice_pop_view* out);
+/* Steps completed on the device. */
+int64_t spice_steps(spice_ctx* ctx);
 /* Spike ids of `pop` from `age` steps ago (age 0 = last step), ascending.
  * Reference neuron_population.h:147-153. */
 int spice_spikes(spice_ctx* ctx, int64_t pop, int64_t age, int32_t* out, int64_t cap,

@@ -65,6 +65,9 @@ def runtime() -> C.CDLL:
         L.spice_ctx_destroy.argtypes = [_vp]
         L.spice_ctx_set_delivery.argtypes = [_vp, C.c_int]
         L.spice_ctx_set_timing.argtypes = [_vp, C.c_int]
+        L.spice_ctx_set_tile.argtypes = [_vp, _i64]
+        L.spice_ctx_tile.restype = _i64
+        L.spice_ctx_tile.argtypes = [_vp]
         L.spice_ctx_stream.restype = _vp
         L.spice_ctx_stream.argtypes = [_vp]
         L.spice_ctx_sync.argtypes = [_vp]
@@ -81,6 +84,8 @@ def runtime() -> C.CDLL:
         L.spice_proj_download.argtypes = [_vp, _i64, _vp, _vp]
         L.spice_block_end.argtypes = [_vp]
         L.spice_spikes.argtypes = [_vp, _i64, _i64, _vp, _i64, C.POINTER(_i64)]
+        L.spice_steps.restype = _i64
+        L.spice_steps.argtypes = [_vp]
         L.spice_history_len.restype = _i64
         L.spice_history_len.argtypes = [_vp]
         L.spice_rng_jump.argtypes = [_u64, _u64, _u64, C.POINTER(_u64), C.POINTER(_u64)]
@@ -198,7 +203,7 @@ class Network:
 
     @classmethod
     def brunel(cls, nP, nE, nI, p, rate, w_ex, w_in, dt=1e-4, delay=15e-4, seeds=(1337,),
-               mode=SPICE_DELIVER_PUSH):
+               mode=SPICE_DELIVER_PULL):
         L = workloads()
         s = (C.c_uint32 * len(seeds))(*seeds)
         h = L.wl_brunel_create(nP, nE, nI, p, rate, w_ex, w_in, dt, delay, s, len(seeds), mode)
@@ -206,7 +211,7 @@ class Network:
 
     @classmethod
     def coba(cls, nE, nI, p, w_ex, w_in, dt=1e-4, delay=1e-4, init=1, seeds=(1337,),
-             mode=SPICE_DELIVER_PUSH):
+             mode=SPICE_DELIVER_PULL):
         L = workloads()
         s = (C.c_uint32 * len(seeds))(*seeds)
         h = L.wl_coba_create(nE, nI, p, w_ex, w_in, dt, delay, init, s, len(seeds), mode)

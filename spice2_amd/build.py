@@ -45,7 +45,7 @@ def run(cmd: list[str], cwd: Path = ROOT) -> None:
 
 def build_runtime() -> Path:
     out = LIB / "libspice_hip.so"
-    srcs = [CSRC / "runtime.hip", CSRC / "csr_gen.hip", CSRC / "jump.cpp"]
+    srcs = [CSRC / "runtime.hip", CSRC / "csr_gen.hip", CSRC / "blocked.hip", CSRC / "jump.cpp"]
     run([hipcc(), *HIPFLAGS, "-shared", *map(str, srcs), "-o", str(out)])
     return out
 

@@ -46,6 +46,12 @@ struct Proj {
 	int64_t nnz = 0;
 	uint32_t* counts = nullptr;    // [slots][dst_size]
 	int64_t slots = 0;
+	// Tile view for pull delivery (blocked.hip): 16-bit tile-local targets in
+	// CSR order and, per row, the in-row start of each tile's run.
+	int64_t tile = 0;              // 0: not built
+	int64_t ntile = 0;
+	uint16_t* rt = nullptr;        // [src_size][ntile + 1]
+	uint16_t* btgt = nullptr;      // [nnz], padded to whole 16-byte vectors
 };
 
 // Device-side descriptors, rebuilt when populations/projections change.
@@ -67,11 +73,26 @@ struct ProjDev {
 	int32_t const* src_ring_ids;
 	int32_t const* src_ring_cnt;
 	int64_t src_size;
+	uint16_t const* rt;
+	uint16_t const* btgt;
+	int64_t ntile;
 };
 
-struct TimedSpan {
-	hipEvent_t a, b;
-	int kind;  // 0 update, 1 compact, 2 deliver
+// Pull delivery: one workgroup per (target tile, step).
+struct PullTile {
+	int32_t target;  // index into PullTarget array
+	int32_t tau;     // tile within the target population
+};
+struct PullTarget {
+	int64_t dst_size;
+	int32_t nin;
+	int32_t proj[SPICE_MAX_IN];  // projection ids, connect order
+};
+
+// Events of one timed block: [0] update start, [1] update end = compaction
+// start, [2] compaction end = delivery start, [3] delivery end.
+struct TimedBlock {
+	hipEvent_t ev[4];
 };
 
 }  // namespace spice_rt
@@ -82,12 +103,17 @@ struct spice_ctx {
 	int64_t max_delay = 1;
 	int64_t ring = 0;  // spike-ring and count-ring slots
 	int64_t steps = 0; // completed steps
-	int mode = SPICE_DELIVER_PUSH;
+	int mode = SPICE_DELIVER_PULL;
 	bool timing = false;
 	std::vector<spice_rt::Pop> pops;
 	std::vector<spice_rt::Proj> projs;
 	spice_rt::PopDev* pops_dev = nullptr;
 	spice_rt::ProjDev* projs_dev = nullptr;
+	spice_rt::PullTile* pull_tiles = nullptr;
+	spice_rt::PullTarget* pull_targets = nullptr;
+	int64_t pull_ntiles = 0;
+	int32_t pull_max_in = 0;
+	int64_t tile = 8192;  // targets per pull tile
 	bool dev_dirty = true;
 	int32_t* error = nullptr;                 // device flag
 	unsigned long long* events = nullptr;     // device counters: events, rows
@@ -95,6 +121,6 @@ struct spice_ctx {
 	int32_t pending = 0;
 	int64_t pending_t0 = 0;
 	hipEvent_t upd_begin = nullptr;
-	std::vector<spice_rt::TimedSpan> spans;
+	std::vector<spice_rt::TimedBlock> timed;
 	spice_stats stats{};
 };

@@ -30,6 +30,7 @@ int set_error(int code, std::string const& msg) {
 
 int generate_fixed_probability(spice_ctx* ctx, Proj& pr, int64_t src_size, int64_t dst_size,
                                double p, xoro_state seed);  // csr_gen.hip
+int build_blocked(spice_ctx* ctx, Proj& Q, int64_t tile);  // blocked.hip
 }  // namespace spice_rt
 
 namespace {
@@ -144,8 +145,116 @@ __global__ void __launch_bounds__(256) deliver_push_kernel(ProjDev const* projs,
 	}
 }
 
+// Pull delivery: one workgroup per (target tile, step).  The workgroup owns the
+// event counters of its tile for every incoming projection in LDS, streams the
+// tile's run of each spiking source's row (16-bit local targets, 16-byte vector
+// loads, spread over the lanes by a wave-wide scan of the run lengths), counts
+// with LDS atomics and writes its counters once, without global atomics.
+// Counts are integers, so the result is independent of the order of the atomics
+// and equals the push result exactly.
+// Workgroup L handles tile L / 16 and step L % 16: the steps of one block land on
+// fixed XCDs (blocks are dealt to the 8 XCDs round-robin), so the spike lists and
+// row offsets a step's workgroups all read are fetched into one XCD's L2.
+struct PullArgs {
+	ProjDev const* projs;
+	PullTile const* tiles;
+	PullTarget const* targets;
+	int64_t t0;
+	int32_t nsteps;
+	int64_t ring;
+	int64_t tile;
+	unsigned long long* events;
+};
+
+__global__ void __launch_bounds__(512) deliver_pull_kernel(PullArgs a) {
+	extern __shared__ uint32_t lds_cnt[];  // [nin][tile]
+	int const k = int(blockIdx.x % SPICE_MAX_BLOCK);
+	if (k >= a.nsteps) return;
+	PullTile const pt   = a.tiles[blockIdx.x / SPICE_MAX_BLOCK];
+	PullTarget const& tg = a.targets[pt.target];
+	int64_t const step  = a.t0 + k;
+	int64_t const T0    = int64_t(pt.tau) * a.tile;
+	int64_t const Tn    = min<int64_t>(a.tile, tg.dst_size - T0);
+	int const tid = threadIdx.x, lane = tid % kWave, wave = tid / kWave, nwaves = blockDim.x / kWave;
+	for (int64_t i = tid; i < int64_t(tg.nin) * a.tile; i += blockDim.x) lds_cnt[i] = 0;
+	__syncthreads();
+	unsigned long long ev = 0, rows = 0;
+	int64_t const slot_in = step % a.ring;
+	for (int j = 0; j < tg.nin; j++) {
+		ProjDev const& P      = a.projs[tg.proj[j]];
+		int32_t const nsp     = P.src_ring_cnt[slot_in];
+		int32_t const* ids    = P.src_ring_ids + slot_in * P.src_size;
+		uint32_t* cnt         = lds_cnt + int64_t(j) * a.tile;
+		int64_t const rtw     = P.ntile + 1;
+		if (pt.tau == 0 && tid == 0) rows += uint64_t(nsp);
+		for (int64_t b0 = int64_t(wave) * kWave; b0 < nsp; b0 += int64_t(nwaves) * kWave) {
+			int64_t const sp = b0 + lane;
+			uint64_t lo = 0, hi = 0;
+			if (sp < nsp) {
+				int64_t const src  = ids[sp];
+				int64_t const base = P.offsets[src];
+				lo                 = uint64_t(base) + P.rt[src * rtw + pt.tau];
+				hi                 = uint64_t(base) + P.rt[src * rtw + pt.tau + 1];
+			}
+			ev += hi - lo;
+			uint64_t const v0 = lo >> 3;
+			uint32_t const nv = hi > lo ? uint32_t(((hi + 7) >> 3) - v0) : 0u;
+			uint32_t incl     = nv;
+			for (int off = 1; off < kWave; off <<= 1) {
+				uint32_t const t = __shfl_up(incl, off, kWave);
+				if (lane >= off) incl += t;
+			}
+			uint32_t const total = __shfl(incl, kWave - 1, kWave);
+			uint32_t const excl  = incl - nv;
+			// wave-uniform loop: every lane takes part in the shuffles, also lanes
+			// past the last vector (a shuffle from an inactive lane is undefined)
+			for (uint32_t vb = 0; vb < total; vb += kWave) {
+				uint32_t const v = vb + lane;
+				int q = 0;  // first lane whose inclusive count exceeds v
+				for (int s = 32; s > 0; s >>= 1)
+					if (__shfl(incl, q + s - 1, kWave) <= v) q += s;
+				q                  = min(q, kWave - 1);
+				uint64_t const vq  = __shfl(v0, q, kWave) + (v - __shfl(excl, q, kWave));
+				uint64_t const lq  = __shfl(lo, q, kWave);
+				uint64_t const hq  = __shfl(hi, q, kWave);
+				if (v < total) {
+					uint4 const w           = *reinterpret_cast<uint4 const*>(P.btgt + vq * 8);
+					uint32_t const words[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+					for (int e = 0; e < 8; e++) {
+						uint64_t const idx = vq * 8 + e;
+						if (idx >= lq && idx < hq)
+							atomicAdd(&cnt[(words[e >> 1] >> (16 * (e & 1))) & 0xffffu], 1u);
+					}
+				}
+			}
+		}
+	}
+	__syncthreads();
+	for (int j = 0; j < tg.nin; j++) {
+		ProjDev const& P = a.projs[tg.proj[j]];
+		uint32_t* out    = P.counts + ((step + P.delay) % P.slots) * P.dst_size + T0;
+		uint32_t const* c = lds_cnt + int64_t(j) * a.tile;
+		for (int64_t i = tid; i < Tn; i += blockDim.x) out[i] = c[i];
+	}
+	for (int off = 32; off > 0; off >>= 1) ev += __shfl_xor(ev, off, kWave);
+	if (lane == 0 && ev) atomicAdd(a.events, ev);
+	if (tid == 0 && rows) atomicAdd(a.events + 1, rows);
+}
+
 int refresh_device_descriptors(spice_ctx* ctx) {
 	if (!ctx->dev_dirty) return SPICE_OK;
+	if (ctx->mode == SPICE_DELIVER_PULL) {
+		// tile size: counters of all incoming projections of a tile fit 64 KB of LDS
+		int64_t max_in = 1;
+		for (auto& P : ctx->pops) max_in = std::max<int64_t>(max_in, int64_t(P.incoming.size()));
+		int64_t tile = std::min<int64_t>(ctx->tile, 65536 / (4 * max_in) / 64 * 64);
+		tile         = std::max<int64_t>(tile, 64);
+		ctx->tile    = tile;
+		for (auto& Q : ctx->projs)
+			if (Q.tile != tile)
+				if (int r = build_blocked(ctx, Q, tile)) return r;
+	}
 	std::vector<PopDev> pd;
 	for (auto& P : ctx->pops)
 		pd.push_back({P.bitmap, P.bitmap_stride, P.size, P.ring_ids, P.ring_cnt});
@@ -153,8 +262,34 @@ int refresh_device_descriptors(spice_ctx* ctx) {
 	for (auto& Q : ctx->projs) {
 		Pop const& S = ctx->pops[Q.src];
 		qd.push_back({Q.offsets, Q.neighbors, Q.counts, Q.slots, ctx->pops[Q.dst].size, Q.delay,
-		              S.ring_ids, S.ring_cnt, S.size});
+		              S.ring_ids, S.ring_cnt, S.size, Q.rt, Q.btgt, Q.ntile});
 	}
+	// pull tiles: every population with incoming projections, split into tiles
+	std::vector<PullTarget> pt;
+	std::vector<PullTile> tiles;
+	ctx->pull_max_in = 0;
+	for (auto& P : ctx->pops) {
+		if (P.incoming.empty() || P.size == 0) continue;
+		PullTarget t{};
+		t.dst_size = P.size;
+		t.nin      = int32_t(P.incoming.size());
+		for (size_t j = 0; j < P.incoming.size(); j++) t.proj[j] = int32_t(P.incoming[j]);
+		int32_t const id = int32_t(pt.size());
+		pt.push_back(t);
+		ctx->pull_max_in = std::max(ctx->pull_max_in, t.nin);
+		for (int64_t tau = 0; tau * ctx->tile < P.size; tau++) tiles.push_back({id, int32_t(tau)});
+	}
+	if (ctx->pull_tiles) (void)hipFree(ctx->pull_tiles);
+	if (ctx->pull_targets) (void)hipFree(ctx->pull_targets);
+	ctx->pull_tiles   = nullptr;
+	ctx->pull_targets = nullptr;
+	ctx->pull_ntiles  = int64_t(tiles.size());
+	if (int r = dev_alloc(&ctx->pull_tiles, tiles.size(), false)) return r;
+	if (int r = dev_alloc(&ctx->pull_targets, pt.size(), false)) return r;
+	if (!tiles.empty())
+		SPICE_HIP(hipMemcpy(ctx->pull_tiles, tiles.data(), sizeof(PullTile) * tiles.size(), hipMemcpyHostToDevice));
+	if (!pt.empty())
+		SPICE_HIP(hipMemcpy(ctx->pull_targets, pt.data(), sizeof(PullTarget) * pt.size(), hipMemcpyHostToDevice));
 	if (ctx->pops_dev) (void)hipFree(ctx->pops_dev);
 	if (ctx->projs_dev) (void)hipFree(ctx->projs_dev);
 	ctx->pops_dev  = nullptr;
@@ -231,11 +366,13 @@ void spice_ctx_destroy(spice_ctx* ctx) {
 		(void)hipFree(Q.offsets);
 		(void)hipFree(Q.neighbors);
 		(void)hipFree(Q.counts);
+		(void)hipFree(Q.rt);
+		(void)hipFree(Q.btgt);
 	}
-	for (auto& s : ctx->spans) {
-		(void)hipEventDestroy(s.a);
-		(void)hipEventDestroy(s.b);
-	}
+	(void)hipFree(ctx->pull_tiles);
+	(void)hipFree(ctx->pull_targets);
+	for (auto& t : ctx->timed)
+		for (auto e : t.ev) (void)hipEventDestroy(e);
 	if (ctx->upd_begin) (void)hipEventDestroy(ctx->upd_begin);
 	(void)hipFree(ctx->pops_dev);
 	(void)hipFree(ctx->projs_dev);
@@ -248,12 +385,25 @@ void spice_ctx_destroy(spice_ctx* ctx) {
 int spice_ctx_set_delivery(spice_ctx* ctx, int mode) {
 	if (mode != SPICE_DELIVER_PUSH && mode != SPICE_DELIVER_PULL)
 		return set_error(SPICE_EINVAL, "unknown delivery mode");
-	if (mode == SPICE_DELIVER_PULL)
-		return set_error(SPICE_EINVAL, "SPICE_DELIVER_PULL is not available in this build");
 	if (ctx->pending) return set_error(SPICE_EINVAL, "cannot change delivery mode inside a block");
-	ctx->mode = mode;
+	// push accumulates into counters that the neuron kernel clears; pull overwrites
+	// them and leaves them set, so the mode is fixed once steps have run
+	if (ctx->steps > 0 && mode != ctx->mode)
+		return set_error(SPICE_EINVAL, "the delivery mode can only be changed before the first step");
+	ctx->mode      = mode;
+	ctx->dev_dirty = true;
 	return SPICE_OK;
 }
+
+int spice_ctx_set_tile(spice_ctx* ctx, int64_t tile) {
+	if (tile < 64 || tile > 65536 || tile % 64) return set_error(SPICE_EINVAL, "tile must be a multiple of 64 in [64, 65536]");
+	if (ctx->pending) return set_error(SPICE_EINVAL, "cannot change the tile inside a block");
+	ctx->tile      = tile;
+	ctx->dev_dirty = true;
+	return SPICE_OK;
+}
+
+int64_t spice_ctx_tile(spice_ctx* ctx) { return ctx->tile; }
 
 int spice_ctx_set_timing(spice_ctx* ctx, int enabled) {
 	ctx->timing = enabled != 0;
@@ -269,22 +419,17 @@ int spice_ctx_sync(spice_ctx* ctx) {
 
 int spice_ctx_stats(spice_ctx* ctx, spice_stats* out) {
 	if (int r = spice_ctx_sync(ctx)) return r;
-	for (auto& s : ctx->spans) {
-		float ms = 0;
-		SPICE_HIP(hipEventElapsedTime(&ms, s.a, s.b));
-		if (s.kind == 0) {
-			ctx->stats.update_ms += ms;
-			ctx->stats.update_launches++;
-		} else if (s.kind == 1) {
-			ctx->stats.compact_ms += ms;
-		} else {
-			ctx->stats.deliver_ms += ms;
-			ctx->stats.deliver_launches++;
-		}
-		(void)hipEventDestroy(s.a);
-		(void)hipEventDestroy(s.b);
+	for (auto& t : ctx->timed) {
+		float ms[3] = {0, 0, 0};
+		for (int k = 0; k < 3; k++) SPICE_HIP(hipEventElapsedTime(&ms[k], t.ev[k], t.ev[k + 1]));
+		ctx->stats.update_ms += ms[0];
+		ctx->stats.update_launches++;
+		ctx->stats.compact_ms += ms[1];
+		ctx->stats.deliver_ms += ms[2];
+		ctx->stats.deliver_launches++;
+		for (auto e : t.ev) (void)hipEventDestroy(e);
 	}
-	ctx->spans.clear();
+	ctx->timed.clear();
 	unsigned long long ev[2] = {0, 0};
 	SPICE_HIP(hipMemcpy(ev, ctx->events, sizeof(ev), hipMemcpyDeviceToHost));
 	ctx->stats.events = ev[0];
@@ -496,7 +641,8 @@ int spice_block_begin(spice_ctx* ctx, int32_t nsteps, float const* dt, uint64_t 
 			xoro_state const s = xoro_jump({seeds[2 * k], seeds[2 * k + 1]}, P.base_poly);
 			v.rng[k]           = {s.s0, s.s1};
 		}
-		v.error = ctx->error;
+		v.error        = ctx->error;
+		v.clear_counts = ctx->mode == SPICE_DELIVER_PUSH ? 1 : 0;
 	}
 	ctx->pending    = nsteps;
 	ctx->pending_t0 = ctx->steps;
@@ -512,34 +658,37 @@ int spice_block_end(spice_ctx* ctx) {
 	SPICE_HIP(hipSetDevice(ctx->device));
 	int const n      = ctx->pending;
 	int64_t const t0 = ctx->pending_t0;
-	hipEvent_t e1 = nullptr, e2 = nullptr, e3 = nullptr;
-	if (ctx->timing) {
-		SPICE_HIP(hipEventCreate(&e1));
-		SPICE_HIP(hipEventCreate(&e2));
-		SPICE_HIP(hipEventCreate(&e3));
-		SPICE_HIP(hipEventRecord(e1, ctx->stream));
-		ctx->spans.push_back({ctx->upd_begin, e1, 0});
+	TimedBlock tb{};
+	bool const timed = ctx->timing && ctx->upd_begin;
+	if (timed) {
+		tb.ev[0]       = ctx->upd_begin;
 		ctx->upd_begin = nullptr;
+		for (int k = 1; k < 4; k++) SPICE_HIP(hipEventCreate(&tb.ev[k]));
+		SPICE_HIP(hipEventRecord(tb.ev[1], ctx->stream));
 	}
 	if (!ctx->pops.empty()) {
 		hipLaunchKernelGGL(compact_kernel, dim3(unsigned(ctx->pops.size()), unsigned(n)), dim3(256), 0,
 		                   ctx->stream, ctx->pops_dev, t0, ctx->ring);
 		SPICE_HIP(hipGetLastError());
 	}
-	if (ctx->timing) {
-		SPICE_HIP(hipEventRecord(e2, ctx->stream));
-		ctx->spans.push_back({e1, e2, 1});
-	}
-	if (!ctx->projs.empty()) {
+	if (timed) SPICE_HIP(hipEventRecord(tb.ev[2], ctx->stream));
+	if (!ctx->projs.empty() && ctx->mode == SPICE_DELIVER_PUSH) {
 		if (ctx->projs.size() * size_t(n) > SPICE_MAX_IN * 4 * SPICE_MAX_BLOCK)
 			return set_error(SPICE_EINVAL, "too many projections for one delivery launch");
 		hipLaunchKernelGGL(deliver_push_kernel, dim3(2048), dim3(256), 0, ctx->stream, ctx->projs_dev,
 		                   int(ctx->projs.size()), t0, n, ctx->ring, ctx->events);
 		SPICE_HIP(hipGetLastError());
+	} else if (!ctx->projs.empty() && ctx->pull_ntiles > 0) {
+		PullArgs a{ctx->projs_dev, ctx->pull_tiles, ctx->pull_targets, t0, n, ctx->ring, ctx->tile,
+		           ctx->events};
+		size_t const lds = sizeof(uint32_t) * size_t(ctx->pull_max_in) * size_t(ctx->tile);
+		hipLaunchKernelGGL(deliver_pull_kernel, dim3(unsigned(ctx->pull_ntiles * SPICE_MAX_BLOCK)), dim3(512),
+		                   lds, ctx->stream, a);
+		SPICE_HIP(hipGetLastError());
 	}
-	if (ctx->timing) {
-		SPICE_HIP(hipEventRecord(e3, ctx->stream));
-		ctx->spans.push_back({e2, e3, 2});
+	if (timed) {
+		SPICE_HIP(hipEventRecord(tb.ev[3], ctx->stream));
+		ctx->timed.push_back(tb);
 	}
 	ctx->steps += n;
 	ctx->pending = 0;
@@ -547,6 +696,31 @@ int spice_block_end(spice_ctx* ctx) {
 }
 
 int64_t spice_history_len(spice_ctx* ctx) { return std::min<int64_t>(ctx->steps, ctx->max_delay); }
+
+int64_t spice_steps(spice_ctx* ctx) { return ctx->steps; }
+
+int spice_pop_view_now(spice_ctx* ctx, int64_t pop, spice_pop_view* v) {
+	if (pop < 0 || pop >= int64_t(ctx->pops.size())) return set_error(SPICE_EINVAL, "bad population");
+	if (ctx->pending) return set_error(SPICE_EINVAL, "block still open");
+	Pop& P = ctx->pops[pop];
+	std::memset(v, 0, sizeof(*v));
+	v->state         = P.state;
+	v->size          = P.size;
+	v->words         = P.words;
+	v->draws         = P.draws;
+	v->bitmap        = P.bitmap;
+	v->bitmap_stride = P.bitmap_stride;
+	v->chunk_jump    = reinterpret_cast<spice_rng_state const*>(P.chunk_jump);
+	v->chunk         = P.chunk;
+	v->nin           = int32_t(P.incoming.size());
+	for (size_t j = 0; j < P.incoming.size(); j++) {
+		Proj& Q = ctx->projs[P.incoming[j]];
+		v->in[j] = {Q.counts, Q.slots, P.incoming[j]};
+	}
+	v->error        = ctx->error;
+	v->clear_counts = 1;  // deliveries applied out of band must not be applied again
+	return SPICE_OK;
+}
 
 int spice_spikes(spice_ctx* ctx, int64_t pop, int64_t age, int32_t* out, int64_t cap,
                  int64_t* out_count) {

@@ -23,7 +23,10 @@ def fnv(h, arr):
     return int(O.oracle().orc_fnv1a(h, O._ptr(arr), arr.nbytes))
 
 
-def brunel(rec, mode=S.SPICE_DELIVER_PUSH):
+MODES = {"pull": S.SPICE_DELIVER_PULL, "push": S.SPICE_DELIVER_PUSH}
+
+
+def brunel(rec, mode=S.SPICE_DELIVER_PULL):
     N = rec["N"]
     wex = float(np.uint32(rec["w_ex_bits"]).view(np.float32))
     win = float(np.uint32(rec["w_in_bits"]).view(np.float32))
@@ -39,12 +42,13 @@ def step_record(net, pops=(0, 1, 2)):
     return [len(x) for x in lists], h, lists
 
 
+@pytest.mark.parametrize("mode", ["pull", "push"])
 @pytest.mark.parametrize("name", ["c0", "sweep0", "sweep1", "sweep2"])
-def test_brunel_raster_every_step(name):
+def test_brunel_raster_every_step(name, mode):
     rec = NETS[name]
     z = np.load(G / f"brunel_{name}.npz")
-    print("network", name, flush=True)
-    with brunel(rec) as net:
+    print("network", name, mode, flush=True)
+    with brunel(rec, MODES[mode]) as net:
         for t in range(rec["steps"]):
             net.step()
             cnt, h, _ = step_record(net)
@@ -76,12 +80,13 @@ def test_brunel_c0_full_raster_and_events():
     assert ev == onet.events()
 
 
-def test_blocks_of_steps_equal_single_steps():
+@pytest.mark.parametrize("mode", ["pull", "push"])
+def test_blocks_of_steps_equal_single_steps(mode):
     """300 steps queued and run in blocks of 15 reach the same state and history
     as steps run one by one."""
     rec = NETS["c0"]
     z = np.load(G / "brunel_c0.npz")
-    with brunel(rec) as net:
+    with brunel(rec, MODES[mode]) as net:
         net.step(300)
         for age in range(15):
             t = 299 - age
@@ -112,11 +117,12 @@ def test_irregular_reads():
             assert cnt == z["counts"][t - 1].tolist() and h == int(z["raster_hash"][t - 1]), t
 
 
-def test_coba_network_delay_one():
+@pytest.mark.parametrize("mode", ["pull", "push"])
+def test_coba_network_delay_one(mode):
     vg = NETS["coba_small"]
     z = np.load(G / "coba_small.npz")
     with S.Network.coba(vg["nE"], vg["nI"], vg["p"], vg["w_ex"], vg["w_in"], delay=vg["delay"],
-                        init=vg["init"], seeds=tuple(vg["seeds"])) as net:
+                        init=vg["init"], seeds=tuple(vg["seeds"]), mode=MODES[mode]) as net:
         for t in range(vg["steps"]):
             net.step()
             cnt = [len(net.spikes(0)), len(net.spikes(1))]
@@ -135,6 +141,30 @@ def test_sample_program_output_is_byte_identical():
     assert hashlib.md5(r.stdout).hexdigest() == "dc05acddc0e3fb6da4e490641121fbe9"
 
 
+@pytest.mark.parametrize("tile", [64, 640, 4096, 65536])
+def test_pull_tile_sizes(tile):
+    """Pull delivery gives the reference raster for any tile size (tiles that do
+    not divide the population, a single tile, tiles of 64 targets)."""
+    import ctypes as C
+    rec = NETS["sweep1"]
+    z = np.load(G / "brunel_sweep1.npz")
+    with brunel(rec) as net:
+        S.check(S.runtime().spice_ctx_set_tile(net.context(), tile))
+        for t in range(120):
+            net.step()
+            cnt, h, _ = step_record(net)
+            assert cnt == z["counts"][t].tolist() and h == int(z["raster_hash"][t]), (tile, t)
+
+
+def test_mode_is_fixed_after_first_step():
+    rec = NETS["sweep0"]
+    with brunel(rec) as net:
+        net.step(3)
+        net.sync()
+        with pytest.raises(S.SpiceError, match="before the first step"):
+            S.check(S.runtime().spice_ctx_set_delivery(net.context(), S.SPICE_DELIVER_PUSH))
+
+
 def test_device_errors_surface():
     with pytest.raises(S.SpicePrecondition, match="at least 1dt"):
         S.Network.brunel(100, 80, 20, 0.1, 20.0, 1e-3, -1e-3, delay=0.0)
@@ -147,14 +177,15 @@ def test_device_errors_surface():
 
 
 @pytest.mark.slow
-def test_brunel_n500k_rasters():
+@pytest.mark.parametrize("mode", ["pull", "push"])
+def test_brunel_n500k_rasters(mode):
     """N=500k, p=0.02, 1000 steps: per-step spike counts and raster hashes, and the
     neuron state every 100 steps, against the reference."""
     if "c1" not in NETS:
         pytest.skip("N=500k golden vectors not recorded")
     rec = NETS["c1"]
     z = np.load(G / "brunel_c1.npz")
-    with brunel(rec) as net:
+    with brunel(rec, MODES[mode]) as net:
         for t in range(rec["steps"]):
             net.step()
             cnt, h, _ = step_record(net)
