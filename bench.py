@@ -61,6 +61,59 @@ def traffic_from_profiles():
         return None
 
 
+COPY_BYTES = 1 << 30  # bytes read (and written) by one calibration copy
+
+
+def copy_calibration(reps: int = 5):
+    """A plain device copy of COPY_BYTES, `reps` times: the known-size kernel that
+    scales the FETCH_SIZE / WRITE_SIZE counters of a rocprofv3 --pmc run."""
+    import torch
+    x = torch.empty(COPY_BYTES // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": COPY_BYTES, "reps": reps}), flush=True)
+
+
+def _pmc_rows(folder: Path, counter: str):
+    """(kernel name, value) per dispatch from rocprofv3 counter_collection CSVs."""
+    import csv
+    rows = []
+    for f in sorted(Path(folder).rglob("*counter_collection.csv")):
+        with open(f, newline="") as fh:
+            for r in csv.DictReader(fh):
+                if r.get("Counter_Name") == counter:
+                    rows.append((r.get("Kernel_Name", ""), float(r.get("Counter_Value", "nan"))))
+    return rows
+
+
+def traffic_summary(fetch_run: Path, write_run: Path, calib_fetch: Path, calib_write: Path, out: Path):
+    """Average HBM bytes per delivery launch from two rocprofv3 --pmc runs of this
+    benchmark (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch), each scaled by what
+    the same counter reads for the calibration copy of COPY_BYTES."""
+    def avg(folder, counter, match):
+        v = [x for k, x in _pmc_rows(folder, counter) if match in k]
+        return (sum(v) / len(v), len(v)) if v else (None, 0)
+    cf, _ = avg(calib_fetch, "FETCH_SIZE", "copy")
+    cw, _ = avg(calib_write, "WRITE_SIZE", "copy")
+    f, nf = avg(fetch_run, "FETCH_SIZE", "deliver_pull_kernel")
+    w, nw = avg(write_run, "WRITE_SIZE", "deliver_pull_kernel")
+    kib = COPY_BYTES / 1024.0
+    sf = kib / cf if cf else None
+    sw = kib / cw if cw else None
+    res = {"kernel": "deliver_pull_kernel", "dispatches": [nf, nw],
+           "fetch_kib_raw": f, "write_kib_raw": w,
+           "copy_fetch_kib_raw": cf, "copy_write_kib_raw": cw, "copy_kib_expected": kib,
+           "fetch_scale": sf, "write_scale": sw,
+           "fetch_bytes": f * 1024 * sf if f and sf else None,
+           "write_bytes": w * 1024 * sw if w and sw else None}
+    res["bytes_per_launch"] = (res["fetch_bytes"] + res["write_bytes"]
+                               if res["fetch_bytes"] and res["write_bytes"] else None)
+    Path(out).write_text(json.dumps(res, indent=1) + "\n")
+    print(json.dumps(res), flush=True)
+
+
 def cpu_baseline(net, steps_warm: int, steps_timed: int):
     """The CPU oracle (one core) on the same network, using the device-generated
     CSR (downloaded, checked identical in the tests), timed over a bounded sample."""
@@ -100,13 +153,15 @@ def cpu_baseline(net, steps_warm: int, steps_timed: int):
             "ms_per_step": 1e3 * dt / steps_timed}
 
 
-def run_mode(mode: int, steps: int, warmup: int):
+def run_mode(mode: int, steps: int, warmup: int, tile: int | None = None):
     import spice2_amd as S
 
     t0 = time.perf_counter()
     net = S.Network.brunel(NP, NE, NI, P_CONN, 20.0, W_EX, W_IN, seeds=(1337,), mode=mode)
     st0 = net.stats()
     csr_s = time.perf_counter() - t0
+    if tile:
+        S.check(S.runtime().spice_ctx_set_tile(net.context(), tile))
     net.step(warmup)
     net.sync()
     net.set_timing(True)
@@ -132,15 +187,28 @@ def main():
     ap.add_argument("--mode", choices=["push", "pull"], default="pull")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=300)
+    ap.add_argument("--tile", type=int, default=None, help="targets per pull tile")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run the known-size copy kernel (PMC counter calibration)")
+    ap.add_argument("--traffic-summary", nargs=5, metavar=("FETCH_RUN", "WRITE_RUN", "CALIB_FETCH",
+                                                            "CALIB_WRITE", "OUT"),
+                    help="summarise rocprofv3 --pmc runs into HBM bytes per delivery launch")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
+    if args.traffic_summary:
+        traffic_summary(*map(Path, args.traffic_summary))
+        return
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU design is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: only one GPU is supported; --gpus {args.gpus} is not (sharding by "
+              "target-neuron slice with a spike all-gather is designed, not built)", file=sys.stderr)
         sys.exit(2)
 
     import spice2_amd as S
     mode = S.SPICE_DELIVER_PUSH if args.mode == "push" else S.SPICE_DELIVER_PULL
-    net, r = run_mode(mode, args.steps, args.warmup)
+    net, r = run_mode(mode, args.steps, args.warmup, args.tile)
+    tile_used = S.runtime().spice_ctx_tile(net.context())
 
     ev = r["events"]
     value = ev / r["seconds"]
@@ -170,7 +238,8 @@ def main():
         "data": "synthetic (fixed_probability connectivity generated on the device from seed 1337)",
         "config": {"workload": "Brunel N=500000 (P 250k / E 200k / I 50k), p=0.02, "
                                "w=+0.2/(N p), -1/(N p), dt=1e-4, delay 15 steps, seed 1337",
-                   "delivery": args.mode, "events_timed": ev, "rows_timed": r["rows"],
+                   "delivery": args.mode, "pull_tile": tile_used if args.mode == "pull" else None,
+                   "events_timed": ev, "rows_timed": r["rows"],
                    "csr_generation_s": r["csr_s"], "csr_device_ms": r["csr_device_ms"],
                    "csr_rows_resolved_on_host": r["csr_host_rows"],
                    "update_ms_per_step": r["update_ms"] / args.steps,

@@ -1,6 +1,8 @@
-// Model-specific device code: the fused neuron-update kernel a population of a
-// user neuron type runs once per block of steps.  Instantiated in the user's
-// translation unit (the user's neuron and synapse functors are inlined).
+// Model-specific device code: the fused neuron-update kernels of a group of
+// populations of one user neuron type, launched once per block of steps.
+// Instantiated in the user's translation unit (the user's neuron and synapse
+// functors are inlined).  One launch covers every population of the group, in
+// every instance of the context.
 //
 // Per neuron and step of the block, in the reference's order (snn.cpp:7-28):
 //   1. replay the deliveries of the previous step: for every incoming
@@ -33,6 +35,22 @@ using replay_fn = void (*)(void const* syn, void* neuron, uint32_t k);
 struct replay_slot {
 	void const* obj;  // device copy of the synapse functor
 	replay_fn fn;     // nullptr: the kernel's own synapse type, inlined
+};
+
+// One workgroup's share of a group launch: population `pop` (context id),
+// group member `member`, elements first .. first + blockDim.x - 1 (neurons or
+// chunks, by kernel).
+struct group_entry {
+	int32_t pop;
+	int32_t member;
+	int64_t first;
+};
+
+template <class Neur>
+struct group_args {
+	group_entry const* wg;              // per workgroup (blockIdx.x)
+	Neur const* neurs;                  // per member: the population's functor
+	replay_slot const* const* slots;    // per member: incoming projections, connect order
 };
 
 // The generator a neuron's update() receives: xoroshiro128+ positioned at the
@@ -72,6 +90,7 @@ struct probe_stream {
 	}
 };
 
+// r <- T^n r for the jump polynomial c = x^n mod P: T^n = sum_i c_i T^i.
 __device__ inline void jump_stream(device_stream& r, spice_rng_state c) {
 	UInt a0 = 0, a1 = 0;
 	for (int i = 0; i < 128; i++) {
@@ -116,23 +135,54 @@ __device__ inline void store_state(N const& n, uint32_t* state, int64_t size, in
 	for (int k = 0; k < W; k++) state[k * size + i] = w[k];
 }
 
-template <class Syn, class N>
-__device__ inline void apply_deliveries(N& n, spice_pop_view const& v, replay_slot const* slots,
-                                        int64_t step, int64_t i) {
-	for (int c = 0; c < v.nin; c++) {
-		uint32_t* cp     = v.in[c].counts + (step % v.in[c].slots) * v.size + i;
-		uint32_t const k = *cp;
-		if (k == 0) continue;
-		if (v.clear_counts) *cp = 0;
-		replay_slot const s = slots[c];
-		if constexpr (!std::is_void_v<Syn>) {
-			if (s.fn == nullptr) {
-				Syn const& syn = *static_cast<Syn const*>(s.obj);
-				for (uint32_t r = 0; r < k; r++) syn.deliver(n);
-				continue;
+__device__ inline spice_step const& step_of(spice_block const& b, int32_t inst, int u) {
+	return b.win->steps[int64_t(inst) * SPICE_MAX_WINDOW + b.rel + u];
+}
+
+// Event counts of neuron i for application step `local`, all incoming
+// projections at once (independent loads, issued together).  MaxIn bounds the
+// population's incoming projections (a compile-time size keeps k in registers).
+template <int MaxIn>
+__device__ inline void load_counts(uint32_t (&k)[MaxIn], spice_pop_view const& v, int64_t local, int64_t i,
+                                   bool clear) {
+#pragma unroll
+	for (int c = 0; c < MaxIn; c++) {
+		k[c] = 0;
+		if (c < v.nin) {
+			uint32_t* cp = v.in[c].counts + (local % v.in[c].slots) * v.size + i;
+			k[c]         = *cp;
+			if (clear && k[c]) *cp = 0;
+		}
+	}
+}
+
+// deliver() k[c] times per incoming projection c, in connect order.  Without
+// Mixed every projection has the kernel's own synapse type and is inlined; with
+// Mixed the others are called through their device function pointer on a copy
+// of the neuron, so that the neuron itself never has its address taken (it
+// stays in registers).
+template <class Syn, int MaxIn, bool Mixed, class N>
+__device__ inline void replay(N& n, uint32_t const (&k)[MaxIn], int nin, replay_slot const* slots) {
+#pragma unroll
+	for (int c = 0; c < MaxIn; c++) {
+		if (c < nin && k[c] != 0) {
+			replay_slot const s = slots[c];
+			bool done           = false;
+			if constexpr (!std::is_void_v<Syn>) {
+				if (!Mixed || s.fn == nullptr) {
+					Syn const& syn = *static_cast<Syn const*>(s.obj);
+					for (uint32_t r = 0; r < k[c]; r++) syn.deliver(n);
+					done = true;
+				}
+			}
+			if constexpr (Mixed || std::is_void_v<Syn>) {
+				if (!done) {
+					N tmp = n;
+					s.fn(s.obj, &tmp, k[c]);
+					n = tmp;
+				}
 			}
 		}
-		s.fn(s.obj, &n, k);
 	}
 }
 
@@ -145,78 +195,121 @@ __device__ inline bool call_update(Neur const& neur, typename neuron_traits<Neur
 		return neur.update(dt, rng);
 }
 
-// One thread per neuron (populations whose neurons take no draws).
-template <class Neur, class Syn>
-__global__ void __launch_bounds__(256)
-update_lanes_kernel(spice_pop_view v, spice_block_view b, Neur neur, replay_slot const* slots) {
-	using N            = typename neuron_traits<Neur>::type;
-	int64_t const i    = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-	bool const active  = i < v.size;
-	int const lane     = threadIdx.x % 64;
+// One thread per neuron (populations whose neurons take no draws).  The counts
+// of step u+1 are loaded while step u is computed.
+template <class Neur, class Syn, int MaxIn, bool Mixed>
+__global__ void __launch_bounds__(256) update_lanes_kernel(spice_block b, group_args<Neur> g) {
+	using N                 = typename neuron_traits<Neur>::type;
+	group_entry const e     = g.wg[blockIdx.x];
+	spice_pop_view const v  = b.pops[e.pop];  // a local copy: stores cannot alias it
+	Neur const neur         = g.neurs[e.member];
+	replay_slot const* sl   = g.slots[e.member];
+	int64_t const i         = e.first + threadIdx.x;
+	bool const active       = i < v.size;
+	int const lane          = threadIdx.x % 64;
+	bool const clear        = v.clear_counts != 0;
 	std::conditional_t<std::is_void_v<N>, char, N> n{};
 	if constexpr (!std::is_void_v<N>)
 		if (active) load_state(n, v.state, v.size, i);
+	uint32_t k[MaxIn] = {}, kn[MaxIn] = {};
+	if constexpr (!std::is_void_v<N>) {
+		spice_step const& s0 = step_of(b, v.inst, 0);
+		if (active && s0.local >= 0) load_counts<MaxIn>(k, v, s0.local, i, clear);
+	}
 	for (int u = 0; u < b.nsteps; u++) {
+		spice_step const& st = step_of(b, v.inst, u);
+		if (st.local < 0) break;  // an instance's steps in a window form a prefix
 		bool spiked = false;
+		if constexpr (!std::is_void_v<N>) {
+			if (u + 1 < b.nsteps) {
+				spice_step const& sn = step_of(b, v.inst, u + 1);
+				if (active && sn.local >= 0) load_counts<MaxIn>(kn, v, sn.local, i, clear);
+			}
+		}
 		if (active) {
-			if constexpr (!std::is_void_v<N>) apply_deliveries<Syn>(n, v, slots, b.t0 + u, i);
-			device_stream rng{v.rng[u].s0, v.rng[u].s1, 0};
+			if constexpr (!std::is_void_v<N>) replay<Syn, MaxIn, Mixed>(n, k, v.nin, sl);
+			device_stream rng{st.seed.s0, st.seed.s1, 0};
 			if constexpr (std::is_void_v<N>)
-				spiked = call_update<Neur>(neur, nullptr, b.dt[u], rng);
+				spiked = call_update<Neur>(neur, nullptr, st.dt, rng);
 			else
-				spiked = call_update<Neur>(neur, &n, b.dt[u], rng);
+				spiked = call_update<Neur>(neur, &n, st.dt, rng);
 			if (rng.count != 0) atomicExch(v.error, 1);
 		}
 		uint64_t const bits = __ballot(spiked);
 		if (lane == 0 && (i >> 6) < v.bitmap_stride) v.bitmap[u * v.bitmap_stride + (i >> 6)] = bits;
+		if constexpr (!std::is_void_v<N>) {
+#pragma unroll
+			for (int c = 0; c < MaxIn; c++) k[c] = kn[c];
+		}
 	}
 	if constexpr (!std::is_void_v<N>)
 		if (active) store_state(n, v.state, v.size, i);
 }
 
-// Applies the deliveries counted for `step` (the next step to run) to the
-// stored state, so that host reads see the state the reference has after the
-// previous step; the counters are cleared, so the next update does not apply
-// them again.
-template <class Neur, class Syn>
-__global__ void __launch_bounds__(256)
-apply_pending_kernel(spice_pop_view v, int64_t step, replay_slot const* slots) {
-	using N         = typename neuron_traits<Neur>::type;
-	int64_t const i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-	if (i >= v.size) return;
-	N n;
-	load_state(n, v.state, v.size, i);
-	apply_deliveries<Syn>(n, v, slots, step, i);
-	store_state(n, v.state, v.size, i);
+// Stateless neurons that draw: their spikes depend only on the step's stream,
+// so every (64-neuron chunk, step) pair is independent.  One thread per pair
+// jumps to the chunk's stream position and writes the chunk's spike word.
+// blockIdx.y = step of the block.
+template <class Neur>
+__global__ void __launch_bounds__(256) update_stateless_kernel(spice_block b, group_args<Neur> g) {
+	group_entry const e     = g.wg[blockIdx.x];
+	spice_pop_view const v  = b.pops[e.pop];  // a local copy: stores cannot alias it
+	int const u             = blockIdx.y;
+	if (u >= b.nsteps) return;
+	spice_step const& st = step_of(b, v.inst, u);
+	if (st.local < 0) return;
+	int64_t const c  = e.first + threadIdx.x;
+	int64_t const i0 = c * 64;
+	if (i0 >= v.size) return;
+	Neur const neur = g.neurs[e.member];
+	device_stream rng{st.seed.s0, st.seed.s1, 0};
+	jump_stream(rng, v.chunk_jump[c * 4]);
+	int const n   = int(v.size - i0 < 64 ? v.size - i0 : 64);
+	uint64_t bits = 0;
+	for (int t = 0; t < n; t++) {
+		bool const spiked = call_update<Neur>(neur, nullptr, st.dt, rng);
+		bits |= uint64_t(spiked) << t;
+	}
+	if (rng.count != n * v.draws) atomicExch(v.error, 1);
+	v.bitmap[u * v.bitmap_stride + c] = bits;
 }
 
-// One thread per chunk of 16 consecutive neurons (neurons take `draws` draws
-// each; the thread jumps once to its chunk's stream position per step).
-template <class Neur, class Syn>
-__global__ void __launch_bounds__(256)
-update_chunks_kernel(spice_pop_view v, spice_block_view b, Neur neur, replay_slot const* slots) {
-	using N             = typename neuron_traits<Neur>::type;
-	constexpr int C     = 16;
-	int64_t const chunk = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-	int64_t const i0    = chunk * C;
-	bool const active   = i0 < v.size;
-	int const lane      = threadIdx.x % 64;
+// One thread per chunk of 16 consecutive neurons (stateful neurons that draw:
+// each takes `draws` draws; the thread jumps once to its chunk's stream
+// position per step).
+template <class Neur, class Syn, int MaxIn, bool Mixed>
+__global__ void __launch_bounds__(256) update_chunks_kernel(spice_block b, group_args<Neur> g) {
+	using N                 = typename neuron_traits<Neur>::type;
+	constexpr int C         = 16;
+	group_entry const e     = g.wg[blockIdx.x];
+	spice_pop_view const v  = b.pops[e.pop];  // a local copy: stores cannot alias it
+	Neur const neur         = g.neurs[e.member];
+	replay_slot const* sl   = g.slots[e.member];
+	int64_t const chunk     = e.first + threadIdx.x;
+	int64_t const i0        = chunk * C;
+	bool const active       = i0 < v.size;
+	int const lane          = threadIdx.x % 64;
+	bool const clear        = v.clear_counts != 0;
 	spice_rng_state const jp = active ? v.chunk_jump[chunk] : spice_rng_state{1, 0};
 	for (int u = 0; u < b.nsteps; u++) {
+		spice_step const& st = step_of(b, v.inst, u);
+		if (st.local < 0) break;
 		uint32_t bits = 0;
 		if (active) {
-			device_stream rng{v.rng[u].s0, v.rng[u].s1, 0};
+			device_stream rng{st.seed.s0, st.seed.s1, 0};
 			jump_stream(rng, jp);
 			int64_t const end = i0 + C < v.size ? i0 + C : v.size;
 			for (int64_t i = i0; i < end; i++) {
 				bool spiked;
 				if constexpr (std::is_void_v<N>) {
-					spiked = call_update<Neur>(neur, nullptr, b.dt[u], rng);
+					spiked = call_update<Neur>(neur, nullptr, st.dt, rng);
 				} else {
 					N n;
 					load_state(n, v.state, v.size, i);
-					apply_deliveries<Syn>(n, v, slots, b.t0 + u, i);
-					spiked = call_update<Neur>(neur, &n, b.dt[u], rng);
+					uint32_t k[MaxIn];
+					load_counts<MaxIn>(k, v, st.local, i, clear);
+					replay<Syn, MaxIn, Mixed>(n, k, v.nin, sl);
+					spiked = call_update<Neur>(neur, &n, st.dt, rng);
 					store_state(n, v.state, v.size, i);
 				}
 				if (rng.count != int((i - i0 + 1) * v.draws)) atomicExch(v.error, 1);
@@ -228,6 +321,25 @@ update_chunks_kernel(spice_pop_view v, spice_block_view b, Neur neur, replay_slo
 		word |= __shfl_xor(word, 2, 64);
 		if (active && lane % 4 == 0) v.bitmap[u * v.bitmap_stride + (i0 >> 6)] = word;
 	}
+}
+
+// Applies the deliveries counted for application step `local` (the next step
+// of the population's instance) to the stored state and clears the counters,
+// so that host reads see the state the reference has after the previous step
+// and the next update does not apply them again.
+template <class Neur, class Syn, int MaxIn, bool Mixed>
+__global__ void __launch_bounds__(256)
+apply_pending_kernel(spice_pop_view const* pops, int64_t pop, int64_t local, replay_slot const* slots) {
+	using N                 = typename neuron_traits<Neur>::type;
+	spice_pop_view const v  = pops[pop];
+	int64_t const i         = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (i >= v.size) return;
+	N n;
+	load_state(n, v.state, v.size, i);
+	uint32_t k[MaxIn];
+	load_counts<MaxIn>(k, v, local, i, true);
+	replay<Syn, MaxIn, Mixed>(n, k, v.nin, slots);
+	store_state(n, v.state, v.size, i);
 }
 
 }  // namespace spice::detail

@@ -1,5 +1,5 @@
 // Populations of one user neuron type (API of reference detail/neuron_population.h),
-// backed by device state in libspice_hip.  Also the engine that runs pending steps.
+// backed by device state in libspice_hip, and the engine that runs queued steps.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -11,6 +11,7 @@
 #include <stdexcept>
 #include <string>
 #include <typeindex>
+#include <utility>
 #include <vector>
 
 #include "spice/concepts.h"
@@ -30,55 +31,156 @@ inline void check_hip(hipError_t e, char const* what) {
 
 struct NeuronPopulation;
 
-// Owns the device context; steps are queued by snn::step() and run in blocks
-// of up to min(16, smallest delay) steps when results are needed.
+// Device buffer owned by the host-side engine objects.
+template <class T>
+struct dev_array {
+	T* p     = nullptr;
+	size_t n = 0;
+	dev_array() = default;
+	dev_array(dev_array const&)            = delete;
+	dev_array& operator=(dev_array const&) = delete;
+	~dev_array() { reset(); }
+	void reset() {
+		if (p) (void)hipFree(p);
+		p = nullptr;
+		n = 0;
+	}
+	void assign(std::vector<T> const& h) {
+		reset();
+		check_hip(hipMalloc(&p, sizeof(T) * std::max<size_t>(h.size(), 1)), "hipMalloc");
+		n = h.size();
+		if (n) check_hip(hipMemcpy(p, h.data(), sizeof(T) * n, hipMemcpyHostToDevice), "hipMemcpy");
+	}
+};
+
+// The populations of one neuron type (and one inlined synapse type) in every
+// instance of a context: one kernel launch per block updates all of them.
+struct group_base {
+	virtual ~group_base()                                        = default;
+	virtual void add(NeuronPopulation* p)                        = 0;
+	virtual void rebuild()                                       = 0;
+	virtual void launch(spice_block const& b, hipStream_t s)     = 0;
+};
+
+// Owns the device context.  Steps are queued per instance (snn::step()) and
+// run in windows when results are needed or a window is full.
 struct engine {
 	spice_ctx* ctx = nullptr;
 	std::vector<NeuronPopulation*> pops;  // context population order
-	std::vector<float> pend_dt;
-	std::vector<UInt> pend_seed;  // lo, hi per step
+	std::vector<std::vector<spice_step>> pending;  // per instance
+	std::vector<std::pair<std::type_index, std::unique_ptr<group_base>>> groups;
+	bool regroup = true;
 	Int min_delay = SPICE_MAX_BLOCK;
+	Int window    = SPICE_MAX_WINDOW;  // steps per spice_run() (graphs are cached per length)
+	bool hold     = false;             // a batch queues whole rounds before flushing
+	std::string cb_error;
 
-	explicit engine(Int max_delay) { check_spice(spice_ctx_create(0, max_delay, &ctx)); }
+	explicit engine(Int max_delay) {
+		check_spice(spice_ctx_create(0, max_delay, &ctx));
+		check_spice(spice_ctx_set_update(ctx, &engine::launch_all, this));
+	}
 	~engine() { spice_ctx_destroy(ctx); }
 	engine(engine const&)            = delete;
 	engine& operator=(engine const&) = delete;
 
+	int64_t add_instance(Int max_delay) {
+		int64_t id = 0;
+		check_spice(spice_instance_create(ctx, max_delay, &id));
+		pending.resize(size_t(id) + 1);
+		return id;
+	}
+
 	hipStream_t stream() const { return static_cast<hipStream_t>(spice_ctx_stream(ctx)); }
+
+	void queue(int64_t inst, float dt, UInt128 seed) {
+		spice_step s{};
+		s.dt   = dt;
+		s.seed = {seed.lo, seed.hi};
+		pending[size_t(inst)].push_back(s);
+		if (!hold && Int(pending[size_t(inst)].size()) >= window) flush();
+	}
+
+	size_t most_pending() const {
+		size_t m = 0;
+		for (auto& q : pending) m = std::max(m, q.size());
+		return m;
+	}
+
 	void flush();
+
+	static int launch_all(void* user, spice_block const* b, void* stream) {
+		auto* e = static_cast<engine*>(user);
+		try {
+			for (auto& g : e->groups) g.second->launch(*b, static_cast<hipStream_t>(stream));
+		} catch (std::exception const& x) {
+			e->cb_error = x.what();
+			return 1;
+		}
+		return 0;
+	}
 };
 
 struct NeuronPopulation {
-	virtual ~NeuronPopulation()                       = default;
-	virtual Int size() const                          = 0;
-	virtual void* neurons()                           = 0;
+	virtual ~NeuronPopulation()                          = default;
+	virtual Int size() const                             = 0;
+	virtual void* neurons()                              = 0;
 	virtual std::span<Int32 const> spikes(Int age) const = 0;
-	virtual void plastic()                            = 0;
-	virtual std::span<UInt const> history() const     = 0;
+	virtual void plastic()                               = 0;
+	virtual std::span<UInt const> history() const        = 0;
 
 	// engine hooks
-	virtual void prepare_block()                                                     = 0;
-	virtual void launch(spice_pop_view const& v, spice_block_view const& b, hipStream_t s) = 0;
-	virtual int64_t id() const                                                       = 0;
+	virtual void prepare()                                 = 0;  // before a window
+	virtual std::type_index group_key() const              = 0;
+	virtual std::unique_ptr<group_base> make_group() const = 0;
+	virtual int64_t id() const                             = 0;
 };
 
 inline void engine::flush() {
-	Int done = 0;
-	Int const total = Int(pend_dt.size());
-	if (total == 0) return;
-	for (auto* p : pops) p->prepare_block();
-	std::vector<spice_pop_view> views(std::max<size_t>(pops.size(), 1));
-	while (done < total) {
-		int32_t const n = int32_t(std::min<Int>({total - done, Int(SPICE_MAX_BLOCK), min_delay}));
-		spice_block_view b;
-		check_spice(spice_block_begin(ctx, n, pend_dt.data() + done, pend_seed.data() + 2 * done, &b,
-		                              views.data(), int32_t(views.size())));
-		for (size_t p = 0; p < pops.size(); p++) pops[p]->launch(views[p], b, stream());
-		check_spice(spice_block_end(ctx));
-		done += n;
+	size_t const most = most_pending();
+	if (most == 0) return;
+	for (auto* p : pops) p->prepare();
+	if (regroup) {
+		groups.clear();
+		for (auto* p : pops) {
+			auto const key = p->group_key();
+			auto it        = std::find_if(groups.begin(), groups.end(), [&](auto& g) { return g.first == key; });
+			if (it == groups.end()) {
+				groups.emplace_back(key, p->make_group());
+				it = groups.end() - 1;
+			}
+			it->second->add(p);
+		}
+		for (auto& g : groups) g.second->rebuild();
+		regroup = false;
 	}
-	pend_dt.clear();
-	pend_seed.clear();
+	size_t const ninst = pending.size();
+	std::vector<spice_step> win;
+	size_t done = 0;
+	while (done < most) {
+		int32_t const n = int32_t(std::min<size_t>(most - done, size_t(window)));
+		win.assign(ninst * size_t(n), spice_step{});
+		for (size_t i = 0; i < ninst; i++) {
+			int64_t const base = spice_instance_steps(ctx, int64_t(i));
+			for (int32_t k = 0; k < n; k++) {
+				spice_step& s = win[i * size_t(n) + size_t(k)];
+				if (done + size_t(k) < pending[i].size()) {
+					s       = pending[i][done + size_t(k)];
+					s.local = base + k;
+				} else {
+					s.local = -1;
+				}
+			}
+		}
+		int const r = spice_run(ctx, n, win.data());
+		if (r != SPICE_OK && !cb_error.empty()) {
+			std::string const m = cb_error;
+			cb_error.clear();
+			throw std::runtime_error("spice: " + m);
+		}
+		check_spice(r);
+		done += size_t(n);
+	}
+	for (auto& q : pending) q.clear();
 }
 
 // Incoming projection of a population, as its update kernel replays it.
@@ -86,6 +188,57 @@ struct incoming {
 	std::type_index type;
 	void const* obj;   // device copy of the synapse functor
 	replay_fn fn;      // device address of its replay thunk
+};
+
+template <Neuron Neur>
+class neuron_population;
+
+// Kernel shape of a population: one thread per neuron (no draws), per
+// (64-neuron chunk, step) (stateless neurons that draw), or per 16-neuron chunk.
+enum class update_kind { lanes, stateless, chunks };
+
+template <class Neur, class Syn, update_kind Kind, int MaxIn, bool Mixed>
+struct group final : group_base {
+	std::vector<neuron_population<Neur>*> members;
+	dev_array<group_entry> wg;
+	dev_array<Neur> neurs;
+	dev_array<replay_slot const*> slots;
+	unsigned nwg = 0;
+
+	void add(NeuronPopulation* p) override { members.push_back(static_cast<neuron_population<Neur>*>(p)); }
+
+	void rebuild() override {
+		std::vector<group_entry> w;
+		std::vector<Neur> nh;
+		std::vector<replay_slot const*> sh;
+		int64_t const per = Kind == update_kind::lanes ? 256 : 256;  // elements per workgroup
+		for (size_t m = 0; m < members.size(); m++) {
+			auto* p = members[m];
+			nh.push_back(p->functor());
+			sh.push_back(p->slots_device());
+			int64_t const elems = Kind == update_kind::lanes       ? p->size()
+			                      : Kind == update_kind::stateless ? (p->size() + 63) / 64
+			                                                       : (p->size() + 15) / 16;
+			for (int64_t f = 0; f < elems; f += per) w.push_back({int32_t(p->id()), int32_t(m), f});
+		}
+		wg.assign(w);
+		neurs.assign(nh);
+		slots.assign(sh);
+		nwg = unsigned(w.size());
+	}
+
+	void launch(spice_block const& b, hipStream_t s) override {
+		if (nwg == 0) return;
+		group_args<Neur> const a{wg.p, neurs.p, slots.p};
+		if constexpr (Kind == update_kind::lanes)
+			hipLaunchKernelGGL((update_lanes_kernel<Neur, Syn, MaxIn, Mixed>), dim3(nwg), dim3(256), 0, s, b, a);
+		else if constexpr (Kind == update_kind::stateless)
+			hipLaunchKernelGGL((update_stateless_kernel<Neur>), dim3(nwg, unsigned(b.nsteps)), dim3(256), 0, s, b,
+			                   a);
+		else
+			hipLaunchKernelGGL((update_chunks_kernel<Neur, Syn, MaxIn, Mixed>), dim3(nwg), dim3(256), 0, s, b, a);
+		check_hip(hipGetLastError(), "neuron update launch");
+	}
 };
 
 template <Neuron Neur>
@@ -97,9 +250,10 @@ public:
 	using neuron_t = neuron_traits_t<Neur>;
 	static constexpr bool stateful = StatefulNeuron<Neur>;
 
-	neuron_population(engine* eng, Neur neuron, Int const size, util::seed_seq& seed, float dt) :
-	_eng(eng), _neuron(std::move(neuron)), _size(size) {
+	neuron_population(engine* eng, int64_t inst, Neur neuron, Int const size, util::seed_seq& seed, float dt) :
+	_eng(eng), _inst(inst), _neuron(std::move(neuron)), _size(size) {
 		SPICE_INV(size >= 0);
+		static_assert(std::is_trivially_copyable_v<Neur>, "neuron functors must be trivially copyable");
 		int32_t words = 0;
 		if constexpr (stateful) {
 			static_assert(std::is_trivially_copyable_v<neuron_t>,
@@ -107,7 +261,7 @@ public:
 			words = int32_t((sizeof(neuron_t) + 3) / 4);
 		}
 		_draws = probe_draws(dt);
-		check_spice(spice_pop_create(eng->ctx, size, words, _draws, &_id));
+		check_spice(spice_pop_create_in(eng->ctx, inst, size, words, _draws, &_id));
 		if constexpr (stateful) {
 			// stateful adapters take a seed whether or not they initialise (neuron_population.h:60)
 			util::xoroshiro64_128p rng(seed++);
@@ -121,14 +275,13 @@ public:
 			_host_dirty = true;
 			_host_valid = true;
 		}
-	}
-
-	~neuron_population() override {
-		if (_slots_dev) (void)hipFree(_slots_dev);
+		set_group<void>();
 	}
 
 	Int size() const override { return _size; }
 	int64_t id() const override { return _id; }
+	Neur const& functor() const { return _neuron; }
+	replay_slot const* slots_device() const { return _slots_dev.p; }
 
 	void* neurons() override {
 		if constexpr (stateful)
@@ -182,23 +335,15 @@ public:
 		(void)hipFree(d_fn);
 		_in.push_back({std::type_index(typeid(Syn)), obj, fn});
 		if (_in.size() == 1) {
-			_primary = std::type_index(typeid(Syn));
-			_launch_primary = [this](spice_pop_view const& v, spice_block_view const& b, hipStream_t s) {
-				run<Syn>(v, b, s);
-			};
-			_apply_primary = [this](spice_pop_view const& v, int64_t step, hipStream_t s) {
-				if constexpr (stateful) {
-					unsigned const grid = unsigned((_size + 255) / 256);
-					hipLaunchKernelGGL((apply_pending_kernel<Neur, Syn>), dim3(grid), dim3(256), 0, s, v, step,
-					                   _slots_dev);
-					check_hip(hipGetLastError(), "apply pending deliveries");
-				}
-			};
+			_primary  = std::type_index(typeid(Syn));
+			_reselect = [this] { set_group<Syn>(); };
 		}
+		_reselect();
 		_slots_dirty = true;
+		_eng->regroup = true;
 	}
 
-	void prepare_block() override {
+	void prepare() override {
 		if constexpr (stateful) {
 			if (_host_dirty) {
 				check_spice(spice_pop_upload(_eng->ctx, _id, _host.data(), int32_t(sizeof(neuron_t))));
@@ -209,52 +354,85 @@ public:
 		if (_slots_dirty) {
 			std::vector<replay_slot> h;
 			for (auto& in : _in) h.push_back({in.obj, in.type == _primary ? nullptr : in.fn});
-			if (_slots_dev) (void)hipFree(_slots_dev);
-			_slots_dev = nullptr;
-			check_hip(hipMalloc(&_slots_dev, sizeof(replay_slot) * std::max<size_t>(h.size(), 1)), "hipMalloc");
-			if (!h.empty())
-				check_hip(hipMemcpy(_slots_dev, h.data(), sizeof(replay_slot) * h.size(), hipMemcpyHostToDevice),
-				          "hipMemcpy");
-			_slots_dirty = false;
+			if (h.empty()) h.push_back({nullptr, nullptr});
+			_slots_dev.assign(h);
+			_slots_dirty  = false;
+			_eng->regroup = true;
 		}
 	}
 
-	void launch(spice_pop_view const& v, spice_block_view const& b, hipStream_t s) override {
-		if (_size == 0) return;
-		if (_launch_primary)
-			_launch_primary(v, b, s);
-		else
-			run<void>(v, b, s);
-	}
+	std::type_index group_key() const override { return _group_key; }
+	std::unique_ptr<group_base> make_group() const override { return _make_group(); }
 
-	// Applies the deliveries already counted for the next step (see kernels.h).
+	// Applies the deliveries already counted for the instance's next step.
 	void apply_pending() {
 		if constexpr (stateful) {
 			if (_in.empty() || _size == 0) return;
-			prepare_block();
-			spice_pop_view v;
-			check_spice(spice_pop_view_now(_eng->ctx, _id, &v));
-			int64_t const step = spice_steps(_eng->ctx);
-			_apply_primary(v, step, _eng->stream());
+			prepare();
+			spice_pop_view const* views = nullptr;
+			check_spice(spice_ctx_views(_eng->ctx, &views));
+			_apply(views, spice_instance_steps(_eng->ctx, _inst));
 		}
 	}
 
 private:
 	engine* _eng;
+	int64_t _inst;
 	Neur _neuron;
 	Int _size;
-	int64_t _id   = -1;
+	int64_t _id    = -1;
 	int32_t _draws = 0;
 	std::vector<std::conditional_t<stateful, neuron_t, char>> _host;
 	bool _host_valid = false, _host_dirty = false;
 	mutable std::vector<Int32> _spike_buf;
 	std::vector<incoming> _in;
 	std::vector<std::unique_ptr<void, void (*)(void*)>> _syn_objs;
-	std::type_index _primary = std::type_index(typeid(void));
-	std::function<void(spice_pop_view const&, spice_block_view const&, hipStream_t)> _launch_primary;
-	std::function<void(spice_pop_view const&, int64_t, hipStream_t)> _apply_primary;
-	replay_slot* _slots_dev = nullptr;
-	bool _slots_dirty       = true;
+	std::type_index _primary   = std::type_index(typeid(void));
+	std::type_index _group_key = std::type_index(typeid(void));
+	std::function<std::unique_ptr<group_base>()> _make_group;
+	std::function<void(spice_pop_view const*, int64_t)> _apply;
+	std::function<void()> _reselect;
+	dev_array<replay_slot> _slots_dev;
+	bool _slots_dirty = true;
+
+	// Picks the update kernel: its shape, the number of incoming projections it
+	// unrolls (4 or SPICE_MAX_IN) and whether some of them have a synapse type
+	// other than the inlined one.
+	template <class Syn>
+	void set_group() {
+		bool mixed = false;
+		for (auto& in : _in) mixed = mixed || in.type != _primary;
+		bool const small = _in.size() <= 4;
+		auto pick_kind   = [&]<int MaxIn, bool Mixed>() {
+			auto pick = [&]<update_kind K>() {
+				_group_key  = std::type_index(typeid(group<Neur, Syn, K, MaxIn, Mixed>));
+				_make_group = [] { return std::unique_ptr<group_base>(new group<Neur, Syn, K, MaxIn, Mixed>()); };
+			};
+			if (_draws == 0)
+				pick.template operator()<update_kind::lanes>();
+			else if constexpr (!stateful)
+				pick.template operator()<update_kind::stateless>();
+			else
+				pick.template operator()<update_kind::chunks>();
+			_apply = [this](spice_pop_view const* views, int64_t local) {
+				(void)this;
+				if constexpr (stateful && !std::is_void_v<Syn>) {
+					unsigned const grid = unsigned((_size + 255) / 256);
+					hipLaunchKernelGGL((apply_pending_kernel<Neur, Syn, MaxIn, Mixed>), dim3(grid), dim3(256), 0,
+					                   _eng->stream(), views, _id, local, _slots_dev.p);
+					check_hip(hipGetLastError(), "apply pending deliveries");
+				}
+			};
+		};
+		if (small && !mixed)
+			pick_kind.template operator()<4, false>();
+		else if (small)
+			pick_kind.template operator()<4, true>();
+		else if (!mixed)
+			pick_kind.template operator()<SPICE_MAX_IN, false>();
+		else
+			pick_kind.template operator()<SPICE_MAX_IN, true>();
+	}
 
 	// RNG outputs update() takes per neuron, probed on the host with a default
 	// neuron.  The device kernels check every neuron against it.
@@ -267,21 +445,6 @@ private:
 			(void)_neuron.update(dt, rng);
 		}
 		return rng.count;
-	}
-
-	template <class Syn>
-	void run(spice_pop_view const& v, spice_block_view const& b, hipStream_t s) {
-		if (_draws == 0) {
-			unsigned const grid = unsigned((_size + 255) / 256);
-			hipLaunchKernelGGL((update_lanes_kernel<Neur, Syn>), dim3(grid), dim3(256), 0, s, v, b, _neuron,
-			                   _slots_dev);
-		} else {
-			Int const chunks    = (_size + 15) / 16;
-			unsigned const grid = unsigned((chunks + 255) / 256);
-			hipLaunchKernelGGL((update_chunks_kernel<Neur, Syn>), dim3(grid), dim3(256), 0, s, v, b,
-			                   _neuron, _slots_dev);
-		}
-		check_hip(hipGetLastError(), "neuron update launch");
 	}
 };
 }  // namespace spice::detail

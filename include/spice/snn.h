@@ -4,8 +4,8 @@
 // to, topology, delay, params) and step() keep the reference's signatures and
 // semantics: the same seed consumption, the same per-step Kahan dt and RNG
 // stream, updates in population order and deliveries in connect order.  step()
-// queues the step; queued steps run on the device in blocks when a result is
-// read (population->spikes(), get_neurons()) or when a block is full.
+// queues the step; queued steps run on the device in windows when a result is
+// read (population->spikes(), get_neurons()) or when a window is full.
 //
 // This header must be compiled with hipcc.  It ends by making every following
 // function of the translation unit callable from device code, so that unchanged
@@ -43,20 +43,21 @@
 #include "spice_hip.h"
 
 namespace spice {
+class batch;
+
 class snn {
 public:
 	snn(float const dt, float const max_delay, util::seed_seq seed) :
-	_dt(dt), _max_delay(Int(std::round(max_delay / dt))), _seed(std::move(seed)) {
-		_eng = std::make_unique<detail::engine>(std::max<Int>(_max_delay, 1));
-	}
+	snn(nullptr, dt, max_delay, std::move(seed)) {}
 
 	template <Neuron Neur>
 	detail::neuron_population<Neur>* add_population(Int const size, Neur neur = {}) {
 		_eng->flush();  // steps queued so far ran without this population
-		auto pop = std::make_unique<detail::neuron_population<Neur>>(_eng.get(), std::move(neur), size,
+		auto pop = std::make_unique<detail::neuron_population<Neur>>(_eng.get(), _inst, std::move(neur), size,
 		                                                              _seed, _dt);
 		auto* raw = pop.get();
 		_eng->pops.push_back(raw);
+		_eng->regroup = true;
 		_neurons.push_back(std::move(pop));
 		return raw;
 	}
@@ -91,11 +92,8 @@ public:
 		float const dt = _simtime += _dt;
 		if (_simtime >= 1) _simtime.reset();
 		UInt128 const s = (_seed++).seed();
-		_eng->pend_dt.push_back(dt);
-		_eng->pend_seed.push_back(s.lo);
-		_eng->pend_seed.push_back(s.hi);
+		_eng->queue(_inst, dt, s);
 		_time++;
-		if (Int(_eng->pend_dt.size()) >= std::min<Int>(SPICE_MAX_BLOCK, _eng->min_delay)) _eng->flush();
 	}
 
 	// ---- extensions of this engine (not in the reference API) -------------
@@ -109,18 +107,86 @@ public:
 		_eng->flush();
 		detail::check_spice(spice_ctx_set_delivery(_eng->ctx, mode));
 	}
+	// Replays the neuron-update, compaction and delivery launches of equal
+	// windows from HIP graphs.
+	void set_graphs(bool on) { detail::check_spice(spice_ctx_set_graphs(_eng->ctx, on ? 1 : 0)); }
 	spice_ctx* context() { return _eng->ctx; }
+	int64_t instance() const { return _inst; }
 	Int time() const { return _time; }
 
 private:
+	friend class batch;
+
+	// eng == nullptr: a network with a context of its own
+	snn(std::shared_ptr<detail::engine> eng, float const dt, float const max_delay, util::seed_seq seed) :
+	_dt(dt), _max_delay(Int(std::round(max_delay / dt))), _seed(std::move(seed)), _eng(std::move(eng)) {
+		if (!_eng) _eng = std::make_shared<detail::engine>(std::max<Int>(_max_delay, 1));
+		_inst = _eng->add_instance(std::max<Int>(_max_delay, 1));
+	}
+
 	Int _time = 0;
 	float _dt;
 	Int _max_delay;
 	util::kahan_sum<float> _simtime;
 	util::seed_seq _seed;
-	std::unique_ptr<detail::engine> _eng;
+	std::shared_ptr<detail::engine> _eng;
+	int64_t _inst = 0;
 	std::vector<std::unique_ptr<detail::NeuronPopulation>> _neurons;
 	std::vector<std::unique_ptr<detail::SynapsePopulation>> _synapses;
+};
+
+// Many independent networks on one GPU (an extension; the reference runs one
+// snn per process): parameter sweeps and seed ensembles.  Each network keeps
+// the reference semantics of a stand-alone snn - its own seed chain, dt
+// sequence, populations and projections - while every step launches one
+// neuron-update kernel per neuron type, one compaction and one delivery for
+// all of them.
+class batch {
+public:
+	// max_delay_steps bounds every member network's max_delay / dt.
+	explicit batch(Int max_delay_steps) :
+	_eng(std::make_shared<detail::engine>(std::max<Int>(max_delay_steps, 1))) {}
+
+	snn& add(float const dt, float const max_delay, util::seed_seq seed) {
+		_nets.push_back(std::unique_ptr<snn>(new snn(_eng, dt, max_delay, std::move(seed))));
+		return *_nets.back();
+	}
+
+	// Steps every network n times.
+	void step(Int n = 1) {
+		struct hold_guard {
+			detail::engine& e;
+			explicit hold_guard(detail::engine& x) : e(x) { e.hold = true; }
+			~hold_guard() { e.hold = false; }
+		} guard(*_eng);
+		for (Int i = 0; i < n; i++) {
+			for (auto& s : _nets) s->step();
+			if (Int(_eng->most_pending()) >= _eng->window) _eng->flush();
+		}
+	}
+	// Steps per device window (1 .. SPICE_MAX_WINDOW); with graphs enabled each
+	// window length is captured once and replayed.
+	void set_window(Int steps) {
+		SPICE_PRE(1 <= steps && steps <= SPICE_MAX_WINDOW);
+		_eng->flush();
+		_eng->window = steps;
+	}
+	void sync() {
+		_eng->flush();
+		detail::check_spice(spice_ctx_sync(_eng->ctx));
+	}
+	void set_delivery(int mode) {
+		_eng->flush();
+		detail::check_spice(spice_ctx_set_delivery(_eng->ctx, mode));
+	}
+	void set_graphs(bool on) { detail::check_spice(spice_ctx_set_graphs(_eng->ctx, on ? 1 : 0)); }
+	Int size() const { return Int(_nets.size()); }
+	snn& operator[](Int i) { return *_nets[size_t(i)]; }
+	spice_ctx* context() { return _eng->ctx; }
+
+private:
+	std::shared_ptr<detail::engine> _eng;
+	std::vector<std::unique_ptr<snn>> _nets;
 };
 }  // namespace spice
 

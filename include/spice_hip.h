@@ -53,29 +53,57 @@ typedef struct {
 	int64_t proj;
 } spice_in_view;
 
-/* Everything a neuron-update kernel needs for one population and one block of
- * steps t0 .. t0+nsteps-1.  Filled by spice_block_begin(). */
+/* One population as the neuron-update kernels see it (device-resident, one per
+ * population, indexed by population id; rebuilt when the network changes). */
 typedef struct {
 	uint32_t* state;        /* SoA: 32-bit word w of neuron i at state[w * size + i] */
 	int64_t size;
 	int32_t words;          /* 32-bit words per neuron (sizeof(neuron) rounded up) */
 	int32_t draws;          /* RNG outputs each neuron takes per step (stream offset stride) */
-	uint64_t* bitmap;       /* [nsteps][bitmap_stride] spike bits, bit i%64 of word i/64 */
+	uint64_t* bitmap;       /* [SPICE_MAX_BLOCK][bitmap_stride] spike bits of the running block */
 	int64_t bitmap_stride;
-	spice_rng_state const* chunk_jump; /* per chunk of `chunk` neurons: x^(first*draws) mod P */
-	int32_t chunk;
+	/* per chunk of 16 neurons: x^(stream offset of the chunk's first draw) mod P,
+	 * P the generator's characteristic polynomial (offset counted from the
+	 * instance's first population) */
+	spice_rng_state const* chunk_jump;
+	int32_t inst;           /* instance the population belongs to */
 	int32_t nin;
 	spice_in_view in[SPICE_MAX_IN]; /* connect order */
-	spice_rng_state rng[SPICE_MAX_BLOCK]; /* per step: stream state at this population's first draw */
 	int32_t* error;         /* device flag: nonzero after a failed device-side check */
 	int32_t clear_counts;   /* the update kernel clears the counters it consumes (push mode) */
 } spice_pop_view;
 
+/* One step of one instance inside a window. */
 typedef struct {
-	int64_t t0;
+	float dt;                /* Kahan-compensated dt of the step (numeric.h:9-15) */
+	int32_t pad;
+	int64_t local;           /* the instance's own step index; -1: the instance does not step */
+	spice_rng_state seed;    /* xoroshiro128+ seed of the step's stream (snn.cpp:12) */
+} spice_step;
+
+#define SPICE_MAX_WINDOW 256 /* steps per spice_run() call */
+
+/* The steps of a window, device-resident: steps[inst * SPICE_MAX_WINDOW + k]. */
+typedef struct {
+	spice_step const* steps;
 	int32_t nsteps;
-	float dt[SPICE_MAX_BLOCK]; /* Kahan-compensated dt of each step (numeric.h:9-15) */
-} spice_block_view;
+	int32_t ninst;
+} spice_window;
+
+/* What a neuron-update kernel launch receives for one block of steps
+ * rel .. rel+nsteps-1 of the running window (device pointers). */
+typedef struct {
+	spice_window const* win;
+	spice_pop_view const* pops;
+	int32_t rel;
+	int32_t nsteps;
+} spice_block;
+
+/* Launches the neuron-update kernels of every population for one block, on
+ * `stream` (a hipStream_t), without synchronising or allocating: blocks may be
+ * captured into a HIP graph and replayed.  Returns 0, or nonzero after an error
+ * (the callback keeps its own message). */
+typedef int (*spice_update_fn)(void* user, spice_block const* block, void* stream);
 
 typedef struct {
 	int64_t steps;          /* simulated steps completed on the device */
@@ -100,8 +128,7 @@ int spice_ctx_create(int device, int64_t max_delay, spice_ctx** out);
 void spice_ctx_destroy(spice_ctx* ctx);
 /* Delivery algorithm (default SPICE_DELIVER_PULL); fixed once steps have run. */
 int spice_ctx_set_delivery(spice_ctx* ctx, int mode);
-/* Targets per pull tile (multiple of 64, <= 65536); the runtime lowers it so that
- * the counters of a tile's incoming projections fit 64 KB of LDS. */
+/* Targets per pull tile (multiple of 64, <= 65536; LDS use is 2 bytes per target). */
 int spice_ctx_set_tile(spice_ctx* ctx, int64_t tile);
 int64_t spice_ctx_tile(spice_ctx* ctx);
 int spice_ctx_set_timing(spice_ctx* ctx, int enabled);
@@ -141,31 +168,49 @@ int spice_proj_create_csr(spice_ctx* ctx, int64_t src_pop, int64_t dst_pop,
 int spice_proj_synapses(spice_ctx* ctx, int64_t proj, int64_t* out_count);
 int spice_proj_download(spice_ctx* ctx, int64_t proj, int64_t* offsets, int32_t* neighbors);
 
+/* ---- instances (independent networks sharing one context: batched
+ *      parameter sweeps and served jobs; no reference counterpart - the
+ *      reference runs one snn per process) ------------------------------------
+ * spice_pop_create() adds to instance 0, created on first use with the
+ * context's max_delay; spice_instance_create() adds instances with their own
+ * max_delay (<= the context's).
+ * Each instance has its own RNG seed chain, dt sequence and step count; its
+ * populations and projections never interact with other instances'. */
+int spice_instance_create(spice_ctx* ctx, int64_t max_delay, int64_t* out_inst);
+int spice_pop_create_in(spice_ctx* ctx, int64_t inst, int64_t size, int32_t state_words,
+                        int32_t draws, int64_t* out_pop);
+int64_t spice_instance_steps(spice_ctx* ctx, int64_t inst); /* steps the instance has run */
+int64_t spice_instance_count(spice_ctx* ctx);
+
 /* ---- stepping (replaces snn::step, snn.cpp:7-28) ----------------------------
- * A block runs nsteps <= min(SPICE_MAX_BLOCK, smallest projection delay) steps:
- *   1. spice_block_begin() fills one spice_pop_view per population;
- *   2. the caller launches each population's neuron-update kernel on
- *      spice_ctx_stream() (it applies pending deliveries, updates, and sets
- *      spike bits);
- *   3. spice_block_end() compacts spike bits into ordered spike lists (the
- *      delay ring) and delivers them.
- * seeds: 2 per step (lo, hi) of the step's RNG stream; dt: per step. */
-int spice_block_begin(spice_ctx* ctx, int32_t nsteps, float const* dt, uint64_t const* seeds,
-                      spice_block_view* block, spice_pop_view* pops, int32_t max_pops);
-int spice_block_end(spice_ctx* ctx);
-/* The view of a population outside a block (no RNG fields): used to apply the
- * deliveries already counted for the next step before the host reads neuron
- * state, since the reference applies each step's deliveries at the end of that
- * step (snn.cpp:21-25) while the device applies them at the start of the next. */
-int spice_pop_view_now(spice_ctx* ctx, int64_t pop, spice_pop_view* out);
-/* Steps completed on the device. */
-int64_t spice_steps(spice_ctx* ctx);
-/* Spike ids of `pop` from `age` steps ago (age 0 = last step), ascending.
- * Reference neuron_population.h:147-153. */
+ * spice_run() runs a window of nsteps <= SPICE_MAX_WINDOW steps.  steps is
+ * [instance][nsteps]: for each instance and step, its dt and stream seed, and
+ * whether it steps (an instance's active steps within a window form a prefix).
+ * The window is split into blocks of at most min(SPICE_MAX_BLOCK, smallest
+ * projection delay) steps; per block:
+ *   1. the update callback launches every population's neuron-update kernel
+ *      (each applies the deliveries due, updates, and sets spike bits);
+ *   2. the runtime compacts spike bits into ordered spike lists (the delay
+ *      ring) and delivers them.
+ * The reference order of snn.cpp:7-28 holds exactly: a spike of step s is
+ * delivered at step s+d-1 and applied before the update of step s+d, and a
+ * block never spans a step that depends on a delivery made inside it.
+ * With graphs enabled, the blocks of a window are captured once per window
+ * shape and replayed. */
+int spice_ctx_set_update(spice_ctx* ctx, spice_update_fn fn, void* user);
+int spice_ctx_set_graphs(spice_ctx* ctx, int enabled);
+int spice_run(spice_ctx* ctx, int32_t nsteps, spice_step const* steps);
+/* Device array of population views (valid until the network changes).  Used by
+ * the kernels that apply the deliveries already counted for the next step before
+ * the host reads neuron state: the reference applies each step's deliveries at
+ * the end of that step (snn.cpp:21-25), the device at the start of the next. */
+int spice_ctx_views(spice_ctx* ctx, spice_pop_view const** out);
+/* Spike ids of `pop` from `age` steps ago (age 0 = its instance's last step),
+ * ascending.  Reference neuron_population.h:147-153. */
 int spice_spikes(spice_ctx* ctx, int64_t pop, int64_t age, int32_t* out, int64_t cap,
                  int64_t* out_count);
-/* Number of steps of spike history kept (the delay window). */
-int64_t spice_history_len(spice_ctx* ctx);
+/* Number of steps of spike history kept for the population's instance. */
+int64_t spice_history_len(spice_ctx* ctx, int64_t pop);
 
 /* ---- host utilities (no device work) ------------------------------------ */
 /* xoroshiro128+ state advanced by n outputs (random.h:222-234). */

@@ -82,12 +82,17 @@ def runtime() -> C.CDLL:
         L.spice_proj_create_csr.argtypes = [_vp, _i64, _i64, _vp, _vp, _i64, C.POINTER(_i64)]
         L.spice_proj_synapses.argtypes = [_vp, _i64, C.POINTER(_i64)]
         L.spice_proj_download.argtypes = [_vp, _i64, _vp, _vp]
-        L.spice_block_end.argtypes = [_vp]
         L.spice_spikes.argtypes = [_vp, _i64, _i64, _vp, _i64, C.POINTER(_i64)]
-        L.spice_steps.restype = _i64
-        L.spice_steps.argtypes = [_vp]
         L.spice_history_len.restype = _i64
-        L.spice_history_len.argtypes = [_vp]
+        L.spice_history_len.argtypes = [_vp, _i64]
+        L.spice_instance_create.argtypes = [_vp, _i64, C.POINTER(_i64)]
+        L.spice_instance_steps.restype = _i64
+        L.spice_instance_steps.argtypes = [_vp, _i64]
+        L.spice_instance_count.restype = _i64
+        L.spice_instance_count.argtypes = [_vp]
+        L.spice_pop_create_in.argtypes = [_vp, _i64, _i64, C.c_int32, C.c_int32, C.POINTER(_i64)]
+        L.spice_ctx_set_graphs.argtypes = [_vp, C.c_int]
+        L.spice_run.argtypes = [_vp, C.c_int32, _vp]
         L.spice_rng_jump.argtypes = [_u64, _u64, _u64, C.POINTER(_u64), C.POINTER(_u64)]
         L.spice_rng_jump.restype = None
         L.spice_rng_jump_poly.argtypes = [_u64, C.POINTER(_u64), C.POINTER(_u64)]
@@ -183,6 +188,18 @@ def workloads() -> C.CDLL:
         L.wl_neurons.argtypes = [_vp, C.c_int, _vp, _i64]
         L.wl_context.restype = _vp
         L.wl_context.argtypes = [_vp]
+        L.wl_set_graphs.argtypes = [_vp, C.c_int]
+        L.wl_batch_brunel_create.restype = _vp
+        L.wl_batch_brunel_create.argtypes = [C.c_int, C.POINTER(_i64), C.POINTER(C.c_double),
+                                             C.POINTER(C.c_float), C.POINTER(C.c_uint32), C.c_float,
+                                             C.c_float, C.c_int]
+        L.wl_batch_destroy.argtypes = [_vp]
+        L.wl_batch_step.argtypes = [_vp, _i64]
+        L.wl_batch_sync.argtypes = [_vp]
+        L.wl_batch_set_graphs.argtypes = [_vp, C.c_int]
+        L.wl_batch_spikes.argtypes = [_vp, C.c_int, C.c_int, _i64, _vp, _i64, C.POINTER(_i64)]
+        L.wl_batch_context.restype = _vp
+        L.wl_batch_context.argtypes = [_vp]
         _wl = L
     return _wl
 
@@ -253,6 +270,79 @@ class Network:
 
     def context(self):
         return self.L.wl_context(self.h)
+
+    def stats(self) -> SpiceStats:
+        s = SpiceStats()
+        check(runtime().spice_ctx_stats(self.context(), C.byref(s)))
+        return s
+
+    def set_timing(self, on: bool):
+        check(runtime().spice_ctx_set_timing(self.context(), int(on)))
+
+    def set_graphs(self, on: bool):
+        self._check(self.L.wl_set_graphs(self.h, int(on)))
+
+
+class BrunelBatch:
+    """Many Brunel networks stepped together in one spice::batch (one context).
+
+    Instance i: N[i] neurons (P/E/I = N/2, N*4/10, N/10), probability p[i],
+    Poisson rate rate[i], weights +0.2/(N p) and -1/(N p), seed seeds[i] (two
+    32-bit words)."""
+
+    def __init__(self, N, p, rate, seeds, dt=1e-4, delay=15e-4, mode=SPICE_DELIVER_PULL):
+        self.L = workloads()
+        n = len(N)
+        self.n = n
+        self.N = [int(x) for x in N]
+        Na = (_i64 * n)(*self.N)
+        pa = (C.c_double * n)(*[float(x) for x in p])
+        ra = (C.c_float * n)(*[float(x) for x in rate])
+        sa = (C.c_uint32 * (2 * n))(*[int(w) for s in seeds for w in s])
+        h = self.L.wl_batch_brunel_create(n, Na, pa, ra, sa, dt, delay, mode)
+        if not h:
+            msg = self.L.wl_error().decode()
+            raise (SpicePrecondition if msg.startswith("Assertion failed") else SpiceError)(msg)
+        self.h = h
+
+    def _check(self, r):
+        if r != 0:
+            msg = self.L.wl_error().decode()
+            raise (SpicePrecondition if r == -3 else SpiceError)(msg)
+
+    def close(self):
+        if self.h:
+            self.L.wl_batch_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def step(self, n: int = 1):
+        self._check(self.L.wl_batch_step(self.h, n))
+
+    def sync(self):
+        self._check(self.L.wl_batch_sync(self.h))
+
+    def set_graphs(self, on: bool):
+        self._check(self.L.wl_batch_set_graphs(self.h, int(on)))
+
+    def sizes(self, inst: int):
+        N = self.N[inst]
+        return (N // 2, N * 4 // 10, N // 10)
+
+    def spikes(self, inst: int, pop: int, age: int = 0) -> np.ndarray:
+        out = np.zeros(max(self.sizes(inst)[pop], 1), np.int32)
+        n = _i64()
+        self._check(self.L.wl_batch_spikes(self.h, inst, pop, age, out.ctypes.data_as(_vp), len(out),
+                                           C.byref(n)))
+        return out[: n.value].copy()
+
+    def context(self):
+        return self.L.wl_batch_context(self.h)
 
     def stats(self) -> SpiceStats:
         s = SpiceStats()

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -23,7 +24,17 @@ int set_error(int code, std::string const& msg);
 			                                             hipGetErrorString(e_));            \
 	} while (0)
 
+// An independent network inside a context: its own step count (the ring
+// slots of its populations are indexed by it), spike-history bound and RNG
+// stream offsets.
+struct Instance {
+	int64_t max_delay = 1;
+	int64_t steps     = 0;  // steps run
+	uint64_t draws    = 0;  // stream outputs taken per step by its populations so far
+};
+
 struct Pop {
+	int32_t inst   = 0;
 	int64_t size   = 0;
 	int32_t words  = 0;
 	int32_t draws  = 0;
@@ -32,9 +43,7 @@ struct Pop {
 	int64_t bitmap_stride = 0;
 	int32_t* ring_ids = nullptr;   // [ring][size]
 	int32_t* ring_cnt = nullptr;   // [ring]
-	xoro_state* chunk_jump = nullptr;
-	int32_t chunk = 16;
-	gf2poly base_poly{1, 0};       // x^(stream offset of neuron 0) mod P
+	xoro_state* chunk_jump = nullptr; // per 16 neurons: x^(stream offset in the instance) mod P
 	std::vector<int64_t> incoming; // projection ids, connect order
 };
 
@@ -56,6 +65,7 @@ struct Proj {
 
 // Device-side descriptors, rebuilt when populations/projections change.
 struct PopDev {
+	int64_t inst;
 	uint64_t const* bitmap;
 	int64_t bitmap_stride;
 	int64_t size;
@@ -64,6 +74,7 @@ struct PopDev {
 };
 
 struct ProjDev {
+	int64_t inst;
 	int64_t const* offsets;
 	int32_t const* neighbors;
 	uint32_t* counts;
@@ -84,6 +95,7 @@ struct PullTile {
 	int32_t tau;     // tile within the target population
 };
 struct PullTarget {
+	int64_t inst;
 	int64_t dst_size;
 	int32_t nin;
 	int32_t proj[SPICE_MAX_IN];  // projection ids, connect order
@@ -102,9 +114,10 @@ struct spice_ctx {
 	hipStream_t stream = nullptr;
 	int64_t max_delay = 1;
 	int64_t ring = 0;  // spike-ring and count-ring slots
-	int64_t steps = 0; // completed steps
+	int64_t steps = 0; // steps run (windows), all instances
 	int mode = SPICE_DELIVER_PULL;
 	bool timing = false;
+	std::vector<spice_rt::Instance> insts;
 	std::vector<spice_rt::Pop> pops;
 	std::vector<spice_rt::Proj> projs;
 	spice_rt::PopDev* pops_dev = nullptr;
@@ -117,10 +130,21 @@ struct spice_ctx {
 	bool dev_dirty = true;
 	int32_t* error = nullptr;                 // device flag
 	unsigned long long* events = nullptr;     // device counters: events, rows
-	// pending block
-	int32_t pending = 0;
-	int64_t pending_t0 = 0;
-	hipEvent_t upd_begin = nullptr;
+	spice_pop_view* views_dev = nullptr;    // [pops]
+	// the running window: device table [insts][SPICE_MAX_WINDOW] filled from one
+	// of two pinned staging buffers (the other may still be in flight)
+	spice_window* win_dev = nullptr;
+	spice_step* win_steps_dev = nullptr;
+	int64_t win_cap = 0;                    // instances the tables hold
+	spice_step* win_host[2] = {nullptr, nullptr};
+	hipEvent_t win_copied[2] = {nullptr, nullptr};
+	int win_next = 0;
+	// neuron updates: launched by the caller's callback once per block
+	spice_update_fn update_fn = nullptr;
+	void* update_user = nullptr;
+	// graphs of whole windows, by window length; dropped when the network changes
+	bool graphs = false;
+	std::map<int32_t, hipGraphExec_t> graph_cache;
 	std::vector<spice_rt::TimedBlock> timed;
 	spice_stats stats{};
 };

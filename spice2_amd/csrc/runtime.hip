@@ -8,6 +8,7 @@
 //                      slot = step % ring, ring = max_delay + SPICE_MAX_BLOCK
 //   projection CSR     int64 offsets [src+1], int32 neighbors [nnz]
 //   event counts       uint32 [ring][dst_size] per projection, slot = application step % ring
+// Steps are the population's instance's own step indices (spice_step::local).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -59,13 +60,20 @@ int precondition(char const* file, int line, char const* cond) {
 
 // ------------------------------------------------------------------ kernels
 
+__device__ inline int64_t local_step(spice_window const* win, int64_t inst, int rel) {
+	return win->steps[inst * SPICE_MAX_WINDOW + rel].local;
+}
+
 // Spike bits of one (population, step) -> ascending id list in the ring slot.
-__global__ void __launch_bounds__(256) compact_kernel(PopDev const* pops, int64_t t0, int64_t ring) {
+__global__ void __launch_bounds__(256)
+compact_kernel(PopDev const* pops, spice_window const* win, int rel, int64_t ring) {
 	PopDev const P  = pops[blockIdx.x];
 	int const k     = blockIdx.y;
+	int64_t const t = local_step(win, P.inst, rel + k);
+	if (t < 0) return;
 	int64_t const nw = (P.size + 63) / 64;
 	uint64_t const* bm = P.bitmap + k * P.bitmap_stride;
-	int64_t const slot = (t0 + k) % ring;
+	int64_t const slot = t % ring;
 	int32_t* out       = P.ring_ids + slot * P.size;
 
 	int const tid      = threadIdx.x;
@@ -96,45 +104,31 @@ __global__ void __launch_bounds__(256) compact_kernel(PopDev const* pops, int64_
 	if (tid == 255) P.ring_cnt[slot] = scan[255];
 }
 
-// Push delivery: waves walk (projection, step, spike) items; each wave streams
-// the spike's CSR row and increments the target's event counter for the
-// application step with a device-scope atomic.  Counts are integers, so the
-// result does not depend on the order of the atomics.
-__global__ void __launch_bounds__(256) deliver_push_kernel(ProjDev const* projs, int nproj,
-                                                           int64_t t0, int nsteps, int64_t ring,
+// Push delivery: one workgroup row per (projection, step); its waves walk the
+// step's spikes, each wave streaming one spike's CSR row and incrementing the
+// target's event counter for the application step with a device-scope atomic.
+// Counts are integers, so the result does not depend on the order of the
+// atomics.  blockIdx.y = projection * SPICE_MAX_BLOCK + step.
+constexpr int kPushRowGroups = 16;
+__global__ void __launch_bounds__(256) deliver_push_kernel(ProjDev const* projs, spice_window const* win,
+                                                           int rel, int nsteps, int64_t ring,
                                                            unsigned long long* events) {
-	__shared__ int64_t pref[SPICE_MAX_IN * 4 * SPICE_MAX_BLOCK + 1];
-	int const nitems = nproj * nsteps;
-	if (threadIdx.x == 0) {
-		int64_t acc = 0;
-		for (int j = 0; j < nitems; j++) {
-			int const c = j / nsteps, k = j % nsteps;
-			pref[j]     = acc;
-			acc += projs[c].src_ring_cnt[(t0 + k) % ring];
-		}
-		pref[nitems] = acc;
-	}
-	__syncthreads();
-	int64_t const total  = pref[nitems];
-	int const lane       = threadIdx.x % kWave;
-	int64_t const gwave  = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
-	int64_t const nwaves = int64_t(gridDim.x) * blockDim.x / kWave;
+	int const k = int(blockIdx.y % SPICE_MAX_BLOCK);
+	if (k >= nsteps) return;
+	ProjDev const& P = projs[blockIdx.y / SPICE_MAX_BLOCK];
+	int64_t const t  = local_step(win, P.inst, rel + k);
+	if (t < 0) return;
+	int64_t const slot = t % ring;
+	int64_t const nsp  = P.src_ring_cnt[slot];
+	int32_t const* ids = P.src_ring_ids + slot * P.src_size;
+	uint32_t* cnt      = P.counts + ((t + P.delay) % P.slots) * P.dst_size;
+	int const lane     = threadIdx.x % kWave;
+	int64_t const gw   = int64_t(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
+	int64_t const nw   = int64_t(gridDim.x) * (blockDim.x / kWave);
 	unsigned long long ev = 0, rows = 0;
-	for (int64_t item = gwave; item < total; item += nwaves) {
-		int lo = 0, hi = nitems;  // pref[lo] <= item < pref[hi]
-		while (hi - lo > 1) {
-			int const mid = (lo + hi) / 2;
-			if (pref[mid] <= item)
-				lo = mid;
-			else
-				hi = mid;
-		}
-		int const c = lo / nsteps, k = lo % nsteps;
-		ProjDev const& P  = projs[c];
-		int64_t const idx = item - pref[lo];
-		int32_t const src = P.src_ring_ids[((t0 + k) % ring) * P.src_size + idx];
+	for (int64_t idx = gw; idx < nsp; idx += nw) {
+		int32_t const src = ids[idx];
 		int64_t const a = P.offsets[src], b = P.offsets[src + 1];
-		uint32_t* cnt   = P.counts + ((t0 + k + P.delay) % P.slots) * P.dst_size;
 		for (int64_t e = a + lane; e < b; e += kWave) atomicAdd(cnt + P.neighbors[e], 1u);
 		ev += uint64_t(b - a);
 		rows++;
@@ -145,167 +139,235 @@ __global__ void __launch_bounds__(256) deliver_push_kernel(ProjDev const* projs,
 	}
 }
 
-// Pull delivery: one workgroup per (target tile, step).  The workgroup owns the
-// event counters of its tile for every incoming projection in LDS, streams the
-// tile's run of each spiking source's row (16-bit local targets, 16-byte vector
-// loads, spread over the lanes by a wave-wide scan of the run lengths), counts
-// with LDS atomics and writes its counters once, without global atomics.
-// Counts are integers, so the result is independent of the order of the atomics
-// and equals the push result exactly.
+// Pull delivery: one workgroup per (target tile, step).  For each incoming
+// projection in turn the workgroup counts, in LDS, the events of the spiking
+// sources that land in its tile and then writes the tile's counters once: no
+// global atomics.  Counters are 16 bits, two per LDS word; a target receives at
+// most one event per spiking source of a projection, so they cannot overflow
+// while fewer than 65536 sources spike - larger spike sets are processed in
+// chunks of 65535 whose counts are added into the (workgroup-owned) counters.
+// Each source's run of targets in the tile is read with 16-byte vector loads
+// (8 tile-local targets), four per lane in flight; a wave-wide scan of the run
+// lengths spreads the vectors of 64 sources evenly over the lanes.  Counts are
+// integers, so the result does not depend on the order of the LDS atomics and
+// equals the push result exactly.
 // Workgroup L handles tile L / 16 and step L % 16: the steps of one block land on
-// fixed XCDs (blocks are dealt to the 8 XCDs round-robin), so the spike lists and
-// row offsets a step's workgroups all read are fetched into one XCD's L2.
+// fixed XCDs (workgroups are dealt to the 8 XCDs round-robin), so the spike
+// lists and row data that all workgroups of a step read are cached in one L2.
 struct PullArgs {
 	ProjDev const* projs;
 	PullTile const* tiles;
 	PullTarget const* targets;
-	int64_t t0;
+	spice_window const* win;
+	int32_t rel;
 	int32_t nsteps;
 	int64_t ring;
 	int64_t tile;
 	unsigned long long* events;
 };
 
+constexpr int kPullUnroll = 4;
+
 __global__ void __launch_bounds__(512) deliver_pull_kernel(PullArgs a) {
-	extern __shared__ uint32_t lds_cnt[];  // [nin][tile]
+	extern __shared__ uint32_t lds_cnt[];  // tile / 2 words, two 16-bit counters each
 	int const k = int(blockIdx.x % SPICE_MAX_BLOCK);
 	if (k >= a.nsteps) return;
-	PullTile const pt   = a.tiles[blockIdx.x / SPICE_MAX_BLOCK];
+	PullTile const pt    = a.tiles[blockIdx.x / SPICE_MAX_BLOCK];
 	PullTarget const& tg = a.targets[pt.target];
-	int64_t const step  = a.t0 + k;
-	int64_t const T0    = int64_t(pt.tau) * a.tile;
-	int64_t const Tn    = min<int64_t>(a.tile, tg.dst_size - T0);
+	int64_t const step   = local_step(a.win, tg.inst, a.rel + k);
+	if (step < 0) return;
+	int64_t const T0     = int64_t(pt.tau) * a.tile;
+	int64_t const Tn     = min<int64_t>(a.tile, tg.dst_size - T0);
+	int64_t const words  = (a.tile + 1) / 2;
 	int const tid = threadIdx.x, lane = tid % kWave, wave = tid / kWave, nwaves = blockDim.x / kWave;
-	for (int64_t i = tid; i < int64_t(tg.nin) * a.tile; i += blockDim.x) lds_cnt[i] = 0;
-	__syncthreads();
 	unsigned long long ev = 0, rows = 0;
 	int64_t const slot_in = step % a.ring;
 	for (int j = 0; j < tg.nin; j++) {
-		ProjDev const& P      = a.projs[tg.proj[j]];
-		int32_t const nsp     = P.src_ring_cnt[slot_in];
-		int32_t const* ids    = P.src_ring_ids + slot_in * P.src_size;
-		uint32_t* cnt         = lds_cnt + int64_t(j) * a.tile;
-		int64_t const rtw     = P.ntile + 1;
+		ProjDev const& P   = a.projs[tg.proj[j]];
+		int64_t const nsp  = P.src_ring_cnt[slot_in];
+		int32_t const* ids = P.src_ring_ids + slot_in * P.src_size;
+		int64_t const rtw  = P.ntile + 1;
+		uint32_t* out      = P.counts + ((step + P.delay) % P.slots) * P.dst_size + T0;
 		if (pt.tau == 0 && tid == 0) rows += uint64_t(nsp);
-		for (int64_t b0 = int64_t(wave) * kWave; b0 < nsp; b0 += int64_t(nwaves) * kWave) {
-			int64_t const sp = b0 + lane;
-			uint64_t lo = 0, hi = 0;
-			if (sp < nsp) {
-				int64_t const src  = ids[sp];
-				int64_t const base = P.offsets[src];
-				lo                 = uint64_t(base) + P.rt[src * rtw + pt.tau];
-				hi                 = uint64_t(base) + P.rt[src * rtw + pt.tau + 1];
-			}
-			ev += hi - lo;
-			uint64_t const v0 = lo >> 3;
-			uint32_t const nv = hi > lo ? uint32_t(((hi + 7) >> 3) - v0) : 0u;
-			uint32_t incl     = nv;
-			for (int off = 1; off < kWave; off <<= 1) {
-				uint32_t const t = __shfl_up(incl, off, kWave);
-				if (lane >= off) incl += t;
-			}
-			uint32_t const total = __shfl(incl, kWave - 1, kWave);
-			uint32_t const excl  = incl - nv;
-			// wave-uniform loop: every lane takes part in the shuffles, also lanes
-			// past the last vector (a shuffle from an inactive lane is undefined)
-			for (uint32_t vb = 0; vb < total; vb += kWave) {
-				uint32_t const v = vb + lane;
-				int q = 0;  // first lane whose inclusive count exceeds v
-				for (int s = 32; s > 0; s >>= 1)
-					if (__shfl(incl, q + s - 1, kWave) <= v) q += s;
-				q                  = min(q, kWave - 1);
-				uint64_t const vq  = __shfl(v0, q, kWave) + (v - __shfl(excl, q, kWave));
-				uint64_t const lq  = __shfl(lo, q, kWave);
-				uint64_t const hq  = __shfl(hi, q, kWave);
-				if (v < total) {
-					uint4 const w           = *reinterpret_cast<uint4 const*>(P.btgt + vq * 8);
-					uint32_t const words[4] = {w.x, w.y, w.z, w.w};
+		for (int64_t c0 = 0; c0 == 0 || c0 < nsp; c0 += 65535) {
+			int64_t const c1 = min<int64_t>(nsp, c0 + 65535);
+			for (int64_t i = tid; i < words; i += blockDim.x) lds_cnt[i] = 0;
+			__syncthreads();
+			for (int64_t b0 = c0 + int64_t(wave) * kWave; b0 < c1; b0 += int64_t(nwaves) * kWave) {
+				int64_t const sp = b0 + lane;
+				uint64_t lo = 0, hi = 0;
+				if (sp < c1) {
+					int64_t const src  = ids[sp];
+					int64_t const base = P.offsets[src];
+					lo                 = uint64_t(base) + P.rt[src * rtw + pt.tau];
+					hi                 = uint64_t(base) + P.rt[src * rtw + pt.tau + 1];
+				}
+				ev += hi - lo;
+				uint64_t const v0 = lo >> 3;
+				uint32_t const nv = hi > lo ? uint32_t(((hi + 7) >> 3) - v0) : 0u;
+				uint32_t incl     = nv;
+				for (int off = 1; off < kWave; off <<= 1) {
+					uint32_t const t = __shfl_up(incl, off, kWave);
+					if (lane >= off) incl += t;
+				}
+				uint32_t const total = __shfl(incl, kWave - 1, kWave);
+				uint32_t const excl  = incl - nv;
+				// wave-uniform loop: every lane takes part in the shuffles (a
+				// shuffle from an inactive lane is undefined)
+				for (uint32_t vb = 0; vb < total; vb += kPullUnroll * kWave) {
+					uint64_t vq[kPullUnroll], lq[kPullUnroll], hq[kPullUnroll];
+					uint4 w[kPullUnroll];
 #pragma unroll
-					for (int e = 0; e < 8; e++) {
-						uint64_t const idx = vq * 8 + e;
-						if (idx >= lq && idx < hq)
-							atomicAdd(&cnt[(words[e >> 1] >> (16 * (e & 1))) & 0xffffu], 1u);
+					for (int u = 0; u < kPullUnroll; u++) {
+						uint32_t const v = vb + u * kWave + lane;
+						int q            = 0;  // first lane whose inclusive count exceeds v
+						for (int s = 32; s > 0; s >>= 1)
+							if (__shfl(incl, q + s - 1, kWave) <= v) q += s;
+						q     = min(q, kWave - 1);
+						vq[u] = __shfl(v0, q, kWave) + (v - __shfl(excl, q, kWave));
+						lq[u] = __shfl(lo, q, kWave);
+						hq[u] = __shfl(hi, q, kWave);
+					}
+#pragma unroll
+					for (int u = 0; u < kPullUnroll; u++)
+						if (vb + u * kWave + lane < total)
+							w[u] = *reinterpret_cast<uint4 const*>(P.btgt + vq[u] * 8);
+#pragma unroll
+					for (int u = 0; u < kPullUnroll; u++) {
+						if (vb + u * kWave + lane >= total) continue;
+						uint32_t const wd[4] = {w[u].x, w[u].y, w[u].z, w[u].w};
+#pragma unroll
+						for (int e = 0; e < 8; e++) {
+							uint64_t const idx = vq[u] * 8 + e;
+							if (idx >= lq[u] && idx < hq[u]) {
+								uint32_t const t = (wd[e >> 1] >> (16 * (e & 1))) & 0xffffu;
+								atomicAdd(&lds_cnt[t >> 1], 1u << (16 * (t & 1)));
+							}
+						}
 					}
 				}
 			}
+			__syncthreads();
+			for (int64_t i = tid; i < Tn; i += blockDim.x) {
+				uint32_t const c = (lds_cnt[i >> 1] >> (16 * (i & 1))) & 0xffffu;
+				if (c0 == 0)
+					out[i] = c;
+				else
+					out[i] += c;
+			}
+			__syncthreads();
 		}
-	}
-	__syncthreads();
-	for (int j = 0; j < tg.nin; j++) {
-		ProjDev const& P = a.projs[tg.proj[j]];
-		uint32_t* out    = P.counts + ((step + P.delay) % P.slots) * P.dst_size + T0;
-		uint32_t const* c = lds_cnt + int64_t(j) * a.tile;
-		for (int64_t i = tid; i < Tn; i += blockDim.x) out[i] = c[i];
 	}
 	for (int off = 32; off > 0; off >>= 1) ev += __shfl_xor(ev, off, kWave);
 	if (lane == 0 && ev) atomicAdd(a.events, ev);
 	if (tid == 0 && rows) atomicAdd(a.events + 1, rows);
 }
 
+// Drops the cached window graphs (they hold the old descriptors and grids).
+void drop_graphs(spice_ctx* ctx) {
+	for (auto& g : ctx->graph_cache) (void)hipGraphExecDestroy(g.second);
+	ctx->graph_cache.clear();
+}
+
 int refresh_device_descriptors(spice_ctx* ctx) {
 	if (!ctx->dev_dirty) return SPICE_OK;
+	drop_graphs(ctx);
+	SPICE_HIP(hipStreamSynchronize(ctx->stream));
 	if (ctx->mode == SPICE_DELIVER_PULL) {
-		// tile size: counters of all incoming projections of a tile fit 64 KB of LDS
-		int64_t max_in = 1;
-		for (auto& P : ctx->pops) max_in = std::max<int64_t>(max_in, int64_t(P.incoming.size()));
-		int64_t tile = std::min<int64_t>(ctx->tile, 65536 / (4 * max_in) / 64 * 64);
-		tile         = std::max<int64_t>(tile, 64);
-		ctx->tile    = tile;
 		for (auto& Q : ctx->projs)
-			if (Q.tile != tile)
-				if (int r = build_blocked(ctx, Q, tile)) return r;
+			if (Q.tile != ctx->tile)
+				if (int r = build_blocked(ctx, Q, ctx->tile)) return r;
 	}
 	std::vector<PopDev> pd;
-	for (auto& P : ctx->pops)
-		pd.push_back({P.bitmap, P.bitmap_stride, P.size, P.ring_ids, P.ring_cnt});
+	std::vector<spice_pop_view> views;
+	for (auto& P : ctx->pops) {
+		pd.push_back({P.inst, P.bitmap, P.bitmap_stride, P.size, P.ring_ids, P.ring_cnt});
+		spice_pop_view v;
+		std::memset(&v, 0, sizeof(v));
+		v.state         = P.state;
+		v.size          = P.size;
+		v.words         = P.words;
+		v.draws         = P.draws;
+		v.bitmap        = P.bitmap;
+		v.bitmap_stride = P.bitmap_stride;
+		v.chunk_jump    = reinterpret_cast<spice_rng_state const*>(P.chunk_jump);
+		v.inst          = P.inst;
+		v.nin           = int32_t(P.incoming.size());
+		for (size_t j = 0; j < P.incoming.size(); j++) {
+			Proj& Q = ctx->projs[P.incoming[j]];
+			v.in[j] = {Q.counts, Q.slots, P.incoming[j]};
+		}
+		v.error        = ctx->error;
+		v.clear_counts = ctx->mode == SPICE_DELIVER_PUSH ? 1 : 0;
+		views.push_back(v);
+	}
 	std::vector<ProjDev> qd;
 	for (auto& Q : ctx->projs) {
 		Pop const& S = ctx->pops[Q.src];
-		qd.push_back({Q.offsets, Q.neighbors, Q.counts, Q.slots, ctx->pops[Q.dst].size, Q.delay,
+		qd.push_back({S.inst, Q.offsets, Q.neighbors, Q.counts, Q.slots, ctx->pops[Q.dst].size, Q.delay,
 		              S.ring_ids, S.ring_cnt, S.size, Q.rt, Q.btgt, Q.ntile});
 	}
 	// pull tiles: every population with incoming projections, split into tiles
 	std::vector<PullTarget> pt;
 	std::vector<PullTile> tiles;
-	ctx->pull_max_in = 0;
 	for (auto& P : ctx->pops) {
 		if (P.incoming.empty() || P.size == 0) continue;
 		PullTarget t{};
+		t.inst     = P.inst;
 		t.dst_size = P.size;
 		t.nin      = int32_t(P.incoming.size());
 		for (size_t j = 0; j < P.incoming.size(); j++) t.proj[j] = int32_t(P.incoming[j]);
 		int32_t const id = int32_t(pt.size());
 		pt.push_back(t);
-		ctx->pull_max_in = std::max(ctx->pull_max_in, t.nin);
 		for (int64_t tau = 0; tau * ctx->tile < P.size; tau++) tiles.push_back({id, int32_t(tau)});
 	}
-	if (ctx->pull_tiles) (void)hipFree(ctx->pull_tiles);
-	if (ctx->pull_targets) (void)hipFree(ctx->pull_targets);
+	(void)hipFree(ctx->pull_tiles);
+	(void)hipFree(ctx->pull_targets);
+	(void)hipFree(ctx->pops_dev);
+	(void)hipFree(ctx->projs_dev);
+	(void)hipFree(ctx->views_dev);
 	ctx->pull_tiles   = nullptr;
 	ctx->pull_targets = nullptr;
+	ctx->pops_dev     = nullptr;
+	ctx->projs_dev    = nullptr;
+	ctx->views_dev    = nullptr;
 	ctx->pull_ntiles  = int64_t(tiles.size());
 	if (int r = dev_alloc(&ctx->pull_tiles, tiles.size(), false)) return r;
 	if (int r = dev_alloc(&ctx->pull_targets, pt.size(), false)) return r;
+	if (int r = dev_alloc(&ctx->pops_dev, pd.size(), false)) return r;
+	if (int r = dev_alloc(&ctx->projs_dev, qd.size(), false)) return r;
+	if (int r = dev_alloc(&ctx->views_dev, views.size(), false)) return r;
 	if (!tiles.empty())
 		SPICE_HIP(hipMemcpy(ctx->pull_tiles, tiles.data(), sizeof(PullTile) * tiles.size(), hipMemcpyHostToDevice));
 	if (!pt.empty())
 		SPICE_HIP(hipMemcpy(ctx->pull_targets, pt.data(), sizeof(PullTarget) * pt.size(), hipMemcpyHostToDevice));
-	if (ctx->pops_dev) (void)hipFree(ctx->pops_dev);
-	if (ctx->projs_dev) (void)hipFree(ctx->projs_dev);
-	ctx->pops_dev  = nullptr;
-	ctx->projs_dev = nullptr;
-	if (int r = dev_alloc(&ctx->pops_dev, pd.size(), false)) return r;
-	if (int r = dev_alloc(&ctx->projs_dev, qd.size(), false)) return r;
 	if (!pd.empty())
 		SPICE_HIP(hipMemcpy(ctx->pops_dev, pd.data(), sizeof(PopDev) * pd.size(), hipMemcpyHostToDevice));
 	if (!qd.empty())
 		SPICE_HIP(hipMemcpy(ctx->projs_dev, qd.data(), sizeof(ProjDev) * qd.size(), hipMemcpyHostToDevice));
-	// stream offsets: population p's first draw follows all earlier populations' draws
-	uint64_t base = 0;
-	for (auto& P : ctx->pops) {
-		P.base_poly = xoro_jump_poly(base);
-		base += uint64_t(P.size) * uint64_t(P.draws);
+	if (!views.empty())
+		SPICE_HIP(hipMemcpy(ctx->views_dev, views.data(), sizeof(spice_pop_view) * views.size(),
+		                    hipMemcpyHostToDevice));
+	// window tables sized for every instance
+	int64_t const ninst = std::max<int64_t>(int64_t(ctx->insts.size()), 1);
+	if (ninst > ctx->win_cap) {
+		int64_t const cap = std::max<int64_t>(ninst, 2 * ctx->win_cap);
+		for (int b = 0; b < 2; b++) {
+			if (ctx->win_copied[b]) SPICE_HIP(hipEventSynchronize(ctx->win_copied[b]));
+			(void)hipHostFree(ctx->win_host[b]);
+			ctx->win_host[b] = nullptr;
+			SPICE_HIP(hipHostMalloc(reinterpret_cast<void**>(&ctx->win_host[b]),
+			                        sizeof(spice_step) * size_t(cap) * SPICE_MAX_WINDOW));
+		}
+		(void)hipFree(ctx->win_steps_dev);
+		ctx->win_steps_dev = nullptr;
+		if (int r = dev_alloc(&ctx->win_steps_dev, size_t(cap) * SPICE_MAX_WINDOW)) return r;
+		ctx->win_cap = cap;
 	}
+	spice_window const w{ctx->win_steps_dev, 0, int32_t(ctx->insts.size())};
+	if (!ctx->win_dev)
+		if (int r = dev_alloc(&ctx->win_dev, 1, false)) return r;
+	SPICE_HIP(hipMemcpy(ctx->win_dev, &w, sizeof(w), hipMemcpyHostToDevice));
 	ctx->dev_dirty = false;
 	return SPICE_OK;
 }
@@ -324,6 +386,50 @@ int check_device_error(spice_ctx* ctx) {
 	return set_error(SPICE_EDEVICE, "device-side check failed (code " + std::to_string(flag) + ")");
 }
 
+// Launches the blocks of the staged window: per block the caller's neuron
+// updates, spike compaction and delivery.  Nothing here synchronises or
+// allocates, so a window can be captured into a graph.
+int run_blocks(spice_ctx* ctx, int32_t nsteps, int32_t L, bool timed) {
+	for (int32_t rel = 0; rel < nsteps; rel += L) {
+		int32_t const n = std::min(L, nsteps - rel);
+		TimedBlock tb{};
+		if (timed) {
+			for (auto& e : tb.ev) SPICE_HIP(hipEventCreate(&e));
+			SPICE_HIP(hipEventRecord(tb.ev[0], ctx->stream));
+		}
+		if (ctx->update_fn) {
+			spice_block const blk{ctx->win_dev, ctx->views_dev, rel, n};
+			if (ctx->update_fn(ctx->update_user, &blk, ctx->stream) != 0)
+				return set_error(SPICE_EINVAL, "neuron-update launch failed");
+		}
+		if (timed) SPICE_HIP(hipEventRecord(tb.ev[1], ctx->stream));
+		if (!ctx->pops.empty()) {
+			hipLaunchKernelGGL(compact_kernel, dim3(unsigned(ctx->pops.size()), unsigned(n)), dim3(256), 0,
+			                   ctx->stream, ctx->pops_dev, ctx->win_dev, rel, ctx->ring);
+			SPICE_HIP(hipGetLastError());
+		}
+		if (timed) SPICE_HIP(hipEventRecord(tb.ev[2], ctx->stream));
+		if (!ctx->projs.empty() && ctx->mode == SPICE_DELIVER_PUSH) {
+			unsigned const rowsy = unsigned(ctx->projs.size() * SPICE_MAX_BLOCK);
+			hipLaunchKernelGGL(deliver_push_kernel, dim3(kPushRowGroups, rowsy), dim3(256), 0, ctx->stream,
+			                   ctx->projs_dev, ctx->win_dev, rel, n, ctx->ring, ctx->events);
+			SPICE_HIP(hipGetLastError());
+		} else if (!ctx->projs.empty() && ctx->pull_ntiles > 0) {
+			PullArgs a{ctx->projs_dev, ctx->pull_tiles, ctx->pull_targets, ctx->win_dev, rel, n,
+			           ctx->ring, ctx->tile, ctx->events};
+			size_t const lds = sizeof(uint32_t) * size_t((ctx->tile + 1) / 2);
+			hipLaunchKernelGGL(deliver_pull_kernel, dim3(unsigned(ctx->pull_ntiles * SPICE_MAX_BLOCK)), dim3(512),
+			                   lds, ctx->stream, a);
+			SPICE_HIP(hipGetLastError());
+		}
+		if (timed) {
+			SPICE_HIP(hipEventRecord(tb.ev[3], ctx->stream));
+			ctx->timed.push_back(tb);
+		}
+	}
+	return SPICE_OK;
+}
+
 }  // namespace
 
 // ====================================================================== C ABI
@@ -340,6 +446,12 @@ int spice_ctx_create(int device, int64_t max_delay, spice_ctx** out) {
 	ctx->device    = device;
 	ctx->max_delay = max_delay;
 	ctx->ring      = max_delay + SPICE_MAX_BLOCK;
+	for (auto& e : ctx->win_copied) {
+		if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+			delete ctx;
+			return set_error(SPICE_EHIP, "hipEventCreate failed");
+		}
+	}
 	hipError_t e   = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
 	if (e != hipSuccess) {
 		delete ctx;
@@ -373,7 +485,14 @@ void spice_ctx_destroy(spice_ctx* ctx) {
 	(void)hipFree(ctx->pull_targets);
 	for (auto& t : ctx->timed)
 		for (auto e : t.ev) (void)hipEventDestroy(e);
-	if (ctx->upd_begin) (void)hipEventDestroy(ctx->upd_begin);
+	drop_graphs(ctx);
+	for (int b = 0; b < 2; b++) {
+		(void)hipHostFree(ctx->win_host[b]);
+		(void)hipEventDestroy(ctx->win_copied[b]);
+	}
+	(void)hipFree(ctx->win_steps_dev);
+	(void)hipFree(ctx->win_dev);
+	(void)hipFree(ctx->views_dev);
 	(void)hipFree(ctx->pops_dev);
 	(void)hipFree(ctx->projs_dev);
 	(void)hipFree(ctx->error);
@@ -385,7 +504,6 @@ void spice_ctx_destroy(spice_ctx* ctx) {
 int spice_ctx_set_delivery(spice_ctx* ctx, int mode) {
 	if (mode != SPICE_DELIVER_PUSH && mode != SPICE_DELIVER_PULL)
 		return set_error(SPICE_EINVAL, "unknown delivery mode");
-	if (ctx->pending) return set_error(SPICE_EINVAL, "cannot change delivery mode inside a block");
 	// push accumulates into counters that the neuron kernel clears; pull overwrites
 	// them and leaves them set, so the mode is fixed once steps have run
 	if (ctx->steps > 0 && mode != ctx->mode)
@@ -397,7 +515,6 @@ int spice_ctx_set_delivery(spice_ctx* ctx, int mode) {
 
 int spice_ctx_set_tile(spice_ctx* ctx, int64_t tile) {
 	if (tile < 64 || tile > 65536 || tile % 64) return set_error(SPICE_EINVAL, "tile must be a multiple of 64 in [64, 65536]");
-	if (ctx->pending) return set_error(SPICE_EINVAL, "cannot change the tile inside a block");
 	ctx->tile      = tile;
 	ctx->dev_dirty = true;
 	return SPICE_OK;
@@ -439,14 +556,35 @@ int spice_ctx_stats(spice_ctx* ctx, spice_stats* out) {
 	return SPICE_OK;
 }
 
-int spice_pop_create(spice_ctx* ctx, int64_t size, int32_t state_words, int32_t draws,
-                     int64_t* out_pop) {
+int spice_instance_create(spice_ctx* ctx, int64_t max_delay, int64_t* out_inst) {
+	if (max_delay < 1) return precondition("neuron_population.h", 108, "max_delay >= 1");
+	if (max_delay > ctx->max_delay)
+		return set_error(SPICE_EINVAL, "an instance's max_delay may not exceed its context's");
+	Instance I;
+	I.max_delay = max_delay;
+	ctx->insts.push_back(I);
+	ctx->dev_dirty = true;
+	*out_inst      = int64_t(ctx->insts.size()) - 1;
+	return SPICE_OK;
+}
+
+int64_t spice_instance_steps(spice_ctx* ctx, int64_t inst) {
+	if (inst < 0 || inst >= int64_t(ctx->insts.size())) return -1;
+	return ctx->insts[inst].steps;
+}
+
+int64_t spice_instance_count(spice_ctx* ctx) { return int64_t(ctx->insts.size()); }
+
+int spice_pop_create_in(spice_ctx* ctx, int64_t inst, int64_t size, int32_t state_words, int32_t draws,
+                        int64_t* out_pop) {
+	if (inst < 0 || inst >= int64_t(ctx->insts.size())) return set_error(SPICE_EINVAL, "bad instance");
 	if (size < 0) return precondition("neuron_population.h", 35, "size >= 0");
 	if (size >= INT32_MAX) return precondition("topology.cpp", 35, "size < INT32_MAX");
 	if (state_words < 0 || draws < 0) return set_error(SPICE_EINVAL, "negative state words or draws");
-	if (ctx->pending) return set_error(SPICE_EINVAL, "cannot add a population inside a block");
 	SPICE_HIP(hipSetDevice(ctx->device));
+	Instance& I = ctx->insts[inst];
 	Pop P;
+	P.inst          = int32_t(inst);
 	P.size          = size;
 	P.words         = state_words;
 	P.draws         = draws;
@@ -456,10 +594,14 @@ int spice_pop_create(spice_ctx* ctx, int64_t size, int32_t state_words, int32_t 
 	if (int r = dev_alloc(&P.ring_ids, size_t(ctx->ring) * size_t(size), false)) return r;
 	if (int r = dev_alloc(&P.ring_cnt, size_t(ctx->ring))) return r;
 	if (draws > 0) {
-		int64_t const nchunks = (size + P.chunk - 1) / P.chunk;
+		// per chunk of 16 neurons: x^(offset of the chunk's first draw in the
+		// instance's per-step stream) mod P; the instance's populations draw in
+		// creation order (reference snn.cpp:12-15)
+		constexpr int64_t C   = 16;
+		int64_t const nchunks = (size + C - 1) / C;
 		std::vector<xoro_state> host(std::max<int64_t>(nchunks, 1));
-		gf2poly const step = xoro_jump_poly(uint64_t(P.chunk) * uint64_t(draws));
-		gf2poly cur{1, 0};
+		gf2poly const step = xoro_jump_poly(uint64_t(C) * uint64_t(draws));
+		gf2poly cur        = xoro_jump_poly(I.draws);
 		for (int64_t j = 0; j < nchunks; j++) {
 			host[j] = {cur.lo, cur.hi};
 			cur     = gf2_mulmod(cur, step);
@@ -468,10 +610,20 @@ int spice_pop_create(spice_ctx* ctx, int64_t size, int32_t state_words, int32_t 
 		SPICE_HIP(hipMemcpy(P.chunk_jump, host.data(), sizeof(xoro_state) * host.size(),
 		                    hipMemcpyHostToDevice));
 	}
+	I.draws += uint64_t(size) * uint64_t(draws);
 	ctx->pops.push_back(P);
 	ctx->dev_dirty = true;
 	*out_pop       = int64_t(ctx->pops.size()) - 1;
 	return SPICE_OK;
+}
+
+int spice_pop_create(spice_ctx* ctx, int64_t size, int32_t state_words, int32_t draws,
+                     int64_t* out_pop) {
+	if (ctx->insts.empty()) {
+		int64_t inst = 0;
+		if (int r = spice_instance_create(ctx, ctx->max_delay, &inst)) return r;
+	}
+	return spice_pop_create_in(ctx, 0, size, state_words, draws, out_pop);
 }
 
 int64_t spice_pop_size(spice_ctx* ctx, int64_t pop) {
@@ -524,14 +676,15 @@ static int proj_common(spice_ctx* ctx, int64_t src, int64_t dst, int64_t delay, 
 	if (src < 0 || dst < 0 || src >= int64_t(ctx->pops.size()) || dst >= int64_t(ctx->pops.size()))
 		return set_error(SPICE_EINVAL, "bad population id");
 	if (!(delay >= 1)) return precondition("snn.h", 35, "d >= 1 && \"The delay must be at least 1dt.\"");
-	if (!(delay <= ctx->max_delay))
+	if (!(delay <= ctx->insts[ctx->pops[src].inst].max_delay))
 		return precondition("snn.h", 36,
 		                    "d <= _max_delay && \"The delay of a synapse population may not exceed "
 		                    "the maximum delay of the network.\"");
 	if (ctx->pops[dst].words == 0) return set_error(SPICE_EINVAL, "target population has no state");
 	if (ctx->pops[dst].incoming.size() >= SPICE_MAX_IN)
 		return set_error(SPICE_EINVAL, "too many incoming projections for one population");
-	if (ctx->pending) return set_error(SPICE_EINVAL, "cannot connect inside a block");
+	if (ctx->pops[src].inst != ctx->pops[dst].inst)
+		return set_error(SPICE_EINVAL, "a projection must connect populations of one instance");
 	Q.src   = src;
 	Q.dst   = dst;
 	Q.delay = delay;
@@ -606,132 +759,106 @@ int spice_proj_download(spice_ctx* ctx, int64_t proj, int64_t* offsets, int32_t*
 	return SPICE_OK;
 }
 
-int spice_block_begin(spice_ctx* ctx, int32_t nsteps, float const* dt, uint64_t const* seeds,
-                      spice_block_view* block, spice_pop_view* views, int32_t max_pops) {
-	if (ctx->pending) return set_error(SPICE_EINVAL, "a block is already open");
-	int32_t limit = SPICE_MAX_BLOCK;
-	for (auto& Q : ctx->projs) limit = int32_t(std::min<int64_t>(limit, Q.delay));
-	if (nsteps < 1 || nsteps > limit)
-		return set_error(SPICE_EINVAL, "block length must be in [1, min(16, smallest delay)]");
-	if (max_pops < int32_t(ctx->pops.size())) return set_error(SPICE_EINVAL, "view array too small");
+int spice_ctx_set_update(spice_ctx* ctx, spice_update_fn fn, void* user) {
+	ctx->update_fn   = fn;
+	ctx->update_user = user;
+	drop_graphs(ctx);
+	return SPICE_OK;
+}
+
+int spice_ctx_set_graphs(spice_ctx* ctx, int enabled) {
+	ctx->graphs = enabled != 0;
+	if (!ctx->graphs) drop_graphs(ctx);
+	return SPICE_OK;
+}
+
+int spice_ctx_views(spice_ctx* ctx, spice_pop_view const** out) {
 	SPICE_HIP(hipSetDevice(ctx->device));
 	if (int r = refresh_device_descriptors(ctx)) return r;
-
-	block->t0     = ctx->steps;
-	block->nsteps = nsteps;
-	for (int k = 0; k < SPICE_MAX_BLOCK; k++) block->dt[k] = k < nsteps ? dt[k] : 0.0f;
-	for (size_t p = 0; p < ctx->pops.size(); p++) {
-		Pop& P           = ctx->pops[p];
-		spice_pop_view& v = views[p];
-		std::memset(&v, 0, sizeof(v));
-		v.state         = P.state;
-		v.size          = P.size;
-		v.words         = P.words;
-		v.draws         = P.draws;
-		v.bitmap        = P.bitmap;
-		v.bitmap_stride = P.bitmap_stride;
-		v.chunk_jump    = reinterpret_cast<spice_rng_state const*>(P.chunk_jump);
-		v.chunk         = P.chunk;
-		v.nin           = int32_t(P.incoming.size());
-		for (size_t j = 0; j < P.incoming.size(); j++) {
-			Proj& Q     = ctx->projs[P.incoming[j]];
-			v.in[j]     = {Q.counts, Q.slots, P.incoming[j]};
-		}
-		for (int k = 0; k < nsteps; k++) {
-			xoro_state const s = xoro_jump({seeds[2 * k], seeds[2 * k + 1]}, P.base_poly);
-			v.rng[k]           = {s.s0, s.s1};
-		}
-		v.error        = ctx->error;
-		v.clear_counts = ctx->mode == SPICE_DELIVER_PUSH ? 1 : 0;
-	}
-	ctx->pending    = nsteps;
-	ctx->pending_t0 = ctx->steps;
-	if (ctx->timing) {
-		SPICE_HIP(hipEventCreate(&ctx->upd_begin));
-		SPICE_HIP(hipEventRecord(ctx->upd_begin, ctx->stream));
-	}
+	*out = ctx->views_dev;
 	return SPICE_OK;
 }
 
-int spice_block_end(spice_ctx* ctx) {
-	if (!ctx->pending) return set_error(SPICE_EINVAL, "no open block");
+int spice_run(spice_ctx* ctx, int32_t nsteps, spice_step const* steps) {
+	if (nsteps < 1 || nsteps > SPICE_MAX_WINDOW)
+		return set_error(SPICE_EINVAL, "a window has 1 to SPICE_MAX_WINDOW steps");
+	if (ctx->insts.empty()) return set_error(SPICE_EINVAL, "the context has no instance");
+	if (!ctx->update_fn && !ctx->pops.empty())
+		return set_error(SPICE_EINVAL, "no neuron-update callback set (spice_ctx_set_update)");
 	SPICE_HIP(hipSetDevice(ctx->device));
-	int const n      = ctx->pending;
-	int64_t const t0 = ctx->pending_t0;
-	TimedBlock tb{};
-	bool const timed = ctx->timing && ctx->upd_begin;
-	if (timed) {
-		tb.ev[0]       = ctx->upd_begin;
-		ctx->upd_begin = nullptr;
-		for (int k = 1; k < 4; k++) SPICE_HIP(hipEventCreate(&tb.ev[k]));
-		SPICE_HIP(hipEventRecord(tb.ev[1], ctx->stream));
+	if (int r = refresh_device_descriptors(ctx)) return r;
+	int64_t const ninst = int64_t(ctx->insts.size());
+	std::vector<int64_t> active(size_t(ninst), 0);
+	for (int64_t i = 0; i < ninst; i++) {
+		bool on = true;
+		for (int k = 0; k < nsteps; k++) {
+			spice_step const& st = steps[i * nsteps + k];
+			if (st.local < 0) {
+				on = false;
+				continue;
+			}
+			if (!on) return set_error(SPICE_EINVAL, "an instance's steps in a window must form a prefix");
+			if (st.local != ctx->insts[i].steps + k)
+				return set_error(SPICE_EINVAL, "spice_step::local must continue the instance's step count");
+			active[i]++;
+		}
 	}
-	if (!ctx->pops.empty()) {
-		hipLaunchKernelGGL(compact_kernel, dim3(unsigned(ctx->pops.size()), unsigned(n)), dim3(256), 0,
-		                   ctx->stream, ctx->pops_dev, t0, ctx->ring);
-		SPICE_HIP(hipGetLastError());
+	// stage the window: pinned buffer b (its previous copy has completed) -> device
+	int const b = ctx->win_next;
+	ctx->win_next ^= 1;
+	SPICE_HIP(hipEventSynchronize(ctx->win_copied[b]));
+	spice_step* h = ctx->win_host[b];
+	for (int64_t i = 0; i < ninst; i++)
+		std::memcpy(h + i * SPICE_MAX_WINDOW, steps + i * nsteps, sizeof(spice_step) * size_t(nsteps));
+	for (int64_t i = 0; i < ninst; i++)
+		SPICE_HIP(hipMemcpyAsync(ctx->win_steps_dev + i * SPICE_MAX_WINDOW, h + i * SPICE_MAX_WINDOW,
+		                         sizeof(spice_step) * size_t(nsteps), hipMemcpyHostToDevice, ctx->stream));
+	SPICE_HIP(hipEventRecord(ctx->win_copied[b], ctx->stream));
+
+	int32_t L = SPICE_MAX_BLOCK;
+	for (auto& Q : ctx->projs) L = int32_t(std::min<int64_t>(L, Q.delay));
+	if (ctx->graphs && !ctx->timing) {
+		auto it = ctx->graph_cache.find(nsteps);
+		if (it == ctx->graph_cache.end()) {
+			SPICE_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+			int const r = run_blocks(ctx, nsteps, L, false);
+			hipGraph_t g = nullptr;
+			hipError_t const e = hipStreamEndCapture(ctx->stream, &g);
+			if (r) {
+				if (g) (void)hipGraphDestroy(g);
+				return r;
+			}
+			if (e != hipSuccess) return set_error(SPICE_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
+			hipGraphExec_t x = nullptr;
+			hipError_t const e2 = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+			(void)hipGraphDestroy(g);
+			if (e2 != hipSuccess) return set_error(SPICE_EHIP, std::string("graph instantiate: ") + hipGetErrorString(e2));
+			it = ctx->graph_cache.emplace(nsteps, x).first;
+		}
+		SPICE_HIP(hipGraphLaunch(it->second, ctx->stream));
+	} else {
+		if (int r = run_blocks(ctx, nsteps, L, ctx->timing)) return r;
 	}
-	if (timed) SPICE_HIP(hipEventRecord(tb.ev[2], ctx->stream));
-	if (!ctx->projs.empty() && ctx->mode == SPICE_DELIVER_PUSH) {
-		if (ctx->projs.size() * size_t(n) > SPICE_MAX_IN * 4 * SPICE_MAX_BLOCK)
-			return set_error(SPICE_EINVAL, "too many projections for one delivery launch");
-		hipLaunchKernelGGL(deliver_push_kernel, dim3(2048), dim3(256), 0, ctx->stream, ctx->projs_dev,
-		                   int(ctx->projs.size()), t0, n, ctx->ring, ctx->events);
-		SPICE_HIP(hipGetLastError());
-	} else if (!ctx->projs.empty() && ctx->pull_ntiles > 0) {
-		PullArgs a{ctx->projs_dev, ctx->pull_tiles, ctx->pull_targets, t0, n, ctx->ring, ctx->tile,
-		           ctx->events};
-		size_t const lds = sizeof(uint32_t) * size_t(ctx->pull_max_in) * size_t(ctx->tile);
-		hipLaunchKernelGGL(deliver_pull_kernel, dim3(unsigned(ctx->pull_ntiles * SPICE_MAX_BLOCK)), dim3(512),
-		                   lds, ctx->stream, a);
-		SPICE_HIP(hipGetLastError());
-	}
-	if (timed) {
-		SPICE_HIP(hipEventRecord(tb.ev[3], ctx->stream));
-		ctx->timed.push_back(tb);
-	}
-	ctx->steps += n;
-	ctx->pending = 0;
+	for (int64_t i = 0; i < ninst; i++) ctx->insts[i].steps += active[i];
+	ctx->steps += nsteps;
 	return SPICE_OK;
 }
 
-int64_t spice_history_len(spice_ctx* ctx) { return std::min<int64_t>(ctx->steps, ctx->max_delay); }
-
-int64_t spice_steps(spice_ctx* ctx) { return ctx->steps; }
-
-int spice_pop_view_now(spice_ctx* ctx, int64_t pop, spice_pop_view* v) {
-	if (pop < 0 || pop >= int64_t(ctx->pops.size())) return set_error(SPICE_EINVAL, "bad population");
-	if (ctx->pending) return set_error(SPICE_EINVAL, "block still open");
-	Pop& P = ctx->pops[pop];
-	std::memset(v, 0, sizeof(*v));
-	v->state         = P.state;
-	v->size          = P.size;
-	v->words         = P.words;
-	v->draws         = P.draws;
-	v->bitmap        = P.bitmap;
-	v->bitmap_stride = P.bitmap_stride;
-	v->chunk_jump    = reinterpret_cast<spice_rng_state const*>(P.chunk_jump);
-	v->chunk         = P.chunk;
-	v->nin           = int32_t(P.incoming.size());
-	for (size_t j = 0; j < P.incoming.size(); j++) {
-		Proj& Q = ctx->projs[P.incoming[j]];
-		v->in[j] = {Q.counts, Q.slots, P.incoming[j]};
-	}
-	v->error        = ctx->error;
-	v->clear_counts = 1;  // deliveries applied out of band must not be applied again
-	return SPICE_OK;
+int64_t spice_history_len(spice_ctx* ctx, int64_t pop) {
+	if (pop < 0 || pop >= int64_t(ctx->pops.size())) return 0;
+	Instance const& I = ctx->insts[ctx->pops[pop].inst];
+	return std::min<int64_t>(I.steps, I.max_delay);
 }
 
 int spice_spikes(spice_ctx* ctx, int64_t pop, int64_t age, int32_t* out, int64_t cap,
                  int64_t* out_count) {
 	if (pop < 0 || pop >= int64_t(ctx->pops.size())) return set_error(SPICE_EINVAL, "bad population");
-	if (ctx->pending) return set_error(SPICE_EINVAL, "block still open");
-	if (!(0 <= age && age < spice_history_len(ctx)))
+	if (!(0 <= age && age < spice_history_len(ctx, pop)))
 		return precondition("neuron_population.h", 148, "0 <= age && age < _spike_counts.size()");
 	SPICE_HIP(hipSetDevice(ctx->device));
 	if (int r = check_device_error(ctx)) return r;
 	Pop& P             = ctx->pops[pop];
-	int64_t const slot = (ctx->steps - 1 - age) % ctx->ring;
+	int64_t const slot = (ctx->insts[P.inst].steps - 1 - age) % ctx->ring;
 	int32_t cnt        = 0;
 	SPICE_HIP(hipMemcpy(&cnt, P.ring_cnt + slot, sizeof(cnt), hipMemcpyDeviceToHost));
 	*out_count = cnt;

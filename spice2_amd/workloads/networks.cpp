@@ -5,6 +5,7 @@
 // samples/vogels.cpp:10-59) with their floating-point expressions unchanged;
 // the Poisson rate is a member so that swept rates reuse the same struct (at
 // 20 Hz it computes exactly what the sample's constant does).
+#include <cmath>
 #include <cstring>
 #include <exception>
 #include <memory>
@@ -105,6 +106,27 @@ struct network {
 	int kind = 0;  // 0 Brunel (P, E, I lif), 1 conductance (E, I)
 };
 
+// Many Brunel networks in one spice::batch.
+struct brunel_batch {
+	std::unique_ptr<spice::batch> b;
+	std::vector<std::vector<spice::detail::NeuronPopulation*>> pops;  // per instance: P, E, I
+};
+
+template <class Net>
+std::vector<spice::detail::NeuronPopulation*> build_brunel(Net& s, int64_t nP, int64_t nE, int64_t nI, double p,
+                                                          float rate, float wex, float win, float delay) {
+	auto P = s.template add_population<wl::poisson>(nP, {rate});
+	auto E = s.template add_population<wl::lif>(nE);
+	auto I = s.template add_population<wl::lif>(nI);
+	s.template connect<wl::fixed_weight>(P, E, fixed_probability(p), delay, {wex});
+	s.template connect<wl::fixed_weight>(P, I, fixed_probability(p), delay, {wex});
+	s.template connect<wl::fixed_weight>(E, E, fixed_probability(p), delay, {wex});
+	s.template connect<wl::fixed_weight>(E, I, fixed_probability(p), delay, {wex});
+	s.template connect<wl::fixed_weight>(I, E, fixed_probability(p), delay, {win});
+	s.template connect<wl::fixed_weight>(I, I, fixed_probability(p), delay, {win});
+	return {P, E, I};
+}
+
 int fail(std::exception const& e) {
 	g_err = e.what();
 	return dynamic_cast<std::logic_error const*>(&e) ? -3 : -1;
@@ -130,16 +152,7 @@ void* wl_brunel_create(int64_t nP, int64_t nE, int64_t nI, double p, float rate,
 		w->net  = std::make_unique<snn>(dt, delay, make_seed(seeds, nseeds));
 		auto& s = *w->net;
 		s.set_delivery(mode);
-		auto P = s.add_population<wl::poisson>(nP, {rate});
-		auto E = s.add_population<wl::lif>(nE);
-		auto I = s.add_population<wl::lif>(nI);
-		s.connect<wl::fixed_weight>(P, E, fixed_probability(p), delay, {wex});
-		s.connect<wl::fixed_weight>(P, I, fixed_probability(p), delay, {wex});
-		s.connect<wl::fixed_weight>(E, E, fixed_probability(p), delay, {wex});
-		s.connect<wl::fixed_weight>(E, I, fixed_probability(p), delay, {wex});
-		s.connect<wl::fixed_weight>(I, E, fixed_probability(p), delay, {win});
-		s.connect<wl::fixed_weight>(I, I, fixed_probability(p), delay, {win});
-		w->pops = {P, E, I};
+		w->pops = build_brunel(s, nP, nE, nI, p, rate, wex, win, delay);
 		w->kind = 0;
 		return w.release();
 	} catch (std::exception const& e) {
@@ -223,4 +236,79 @@ int wl_neurons(void* h, int pop, void* out, int64_t bytes) {
 }
 
 spice_ctx* wl_context(void* h) { return static_cast<network*>(h)->net->context(); }
+
+int wl_set_graphs(void* h, int on) {
+	try {
+		static_cast<network*>(h)->net->set_graphs(on != 0);
+		return 0;
+	} catch (std::exception const& e) {
+		return fail(e);
+	}
+}
+
+// n Brunel networks in one batch: instance i has N[i] neurons split P/E/I as
+// N/2, N*4/10, N/10, probability p[i], Poisson rate rate[i], weights
+// +0.2/(N p) and -1/(N p), and seed {seeds[2i], seeds[2i+1]}.
+void* wl_batch_brunel_create(int n, int64_t const* N, double const* p, float const* rate,
+                             uint32_t const* seeds, float dt, float delay, int mode) {
+	try {
+		auto w = std::make_unique<brunel_batch>();
+		w->b   = std::make_unique<spice::batch>(Int(std::lround(delay / dt)));
+		w->b->set_delivery(mode);
+		for (int i = 0; i < n; i++) {
+			auto& s         = w->b->add(dt, delay, make_seed(seeds + 2 * i, 2));
+			float const wex = float(0.2 / (double(N[i]) * p[i]));
+			float const win = float(-1.0 / (double(N[i]) * p[i]));
+			w->pops.push_back(build_brunel(s, N[i] / 2, N[i] * 4 / 10, N[i] / 10, p[i], rate[i], wex, win, delay));
+		}
+		return w.release();
+	} catch (std::exception const& e) {
+		fail(e);
+		return nullptr;
+	}
+}
+
+void wl_batch_destroy(void* h) { delete static_cast<brunel_batch*>(h); }
+
+int wl_batch_step(void* h, int64_t n) {
+	try {
+		static_cast<brunel_batch*>(h)->b->step(n);
+		return 0;
+	} catch (std::exception const& e) {
+		return fail(e);
+	}
+}
+
+int wl_batch_sync(void* h) {
+	try {
+		static_cast<brunel_batch*>(h)->b->sync();
+		return 0;
+	} catch (std::exception const& e) {
+		return fail(e);
+	}
+}
+
+int wl_batch_set_graphs(void* h, int on) {
+	try {
+		static_cast<brunel_batch*>(h)->b->set_graphs(on != 0);
+		return 0;
+	} catch (std::exception const& e) {
+		return fail(e);
+	}
+}
+
+int wl_batch_spikes(void* h, int inst, int pop, int64_t age, int32_t* out, int64_t cap, int64_t* count) {
+	try {
+		auto* w = static_cast<brunel_batch*>(h);
+		auto s  = w->pops.at(size_t(inst)).at(size_t(pop))->spikes(age);
+		*count  = int64_t(s.size());
+		if (int64_t(s.size()) > cap) throw std::runtime_error("spike buffer too small");
+		std::memcpy(out, s.data(), sizeof(int32_t) * s.size());
+		return 0;
+	} catch (std::exception const& e) {
+		return fail(e);
+	}
+}
+
+spice_ctx* wl_batch_context(void* h) { return static_cast<brunel_batch*>(h)->b->context(); }
 }
