@@ -153,6 +153,73 @@ def cpu_baseline(net, steps_warm: int, steps_timed: int):
             "ms_per_step": 1e3 * dt / steps_timed}
 
 
+def sweep_instances(n: int, seed: int = 2024):
+    """The batched sweep: N log-uniform in [5000, 100000], p = min(0.1, 2000/N),
+    Poisson rate uniform in [10, 30] Hz, seeds {1337, i} (numpy PCG64(seed))."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    N = np.round(np.exp(rng.uniform(np.log(5e3), np.log(1e5), n))).astype(np.int64)
+    p = np.minimum(0.1, 2000.0 / N)
+    rate = rng.uniform(10.0, 30.0, n)
+    return N, p, rate, [(1337, i) for i in range(n)]
+
+
+def run_batch(n: int, steps: int, window: int, plain_sample: int):
+    """`n` Brunel instances in one batch (one context; windows of `window` steps
+    replayed from HIP graphs; pull tiles autotuned per target-size bucket) against
+    a stratified sample of the same instances run one after another through the
+    plain single-network path.  Returns the batched and plain aggregate synaptic
+    events/s and whether the sampled instances' final spike histories agree."""
+    import spice2_amd as S
+    N, p, rate, seeds = sweep_instances(n)
+    t0 = time.perf_counter()
+    b = S.BrunelBatch(N, p, rate, seeds)
+    setup_s = time.perf_counter() - t0
+    b.set_window(window)
+    b.set_graphs(True)
+    b.step(window)            # warm: one window (captures its graph)
+    b.sync()
+    tiles = b.autotune()
+    b.step(window)            # re-capture with the tuned tiles
+    b.sync()
+    e0 = b.stats().events
+    t1 = time.perf_counter()
+    b.step(steps)
+    b.sync()
+    dt_b = time.perf_counter() - t1
+    ev_b = b.stats().events - e0
+    total_steps = steps + 2 * window
+    order = np.argsort(N)
+    sample = [int(order[k]) for k in np.linspace(0, n - 1, plain_sample).round().astype(int)]
+    ev_p = 0
+    dt_p = 0.0
+    same = True
+    for i in sample:
+        wex = float(np.float32(0.2 / (N[i] * p[i])))
+        win = float(np.float32(-1.0 / (N[i] * p[i])))
+        with S.Network.brunel(int(N[i]) // 2, int(N[i]) * 4 // 10, int(N[i]) // 10, float(p[i]),
+                              float(rate[i]), wex, win, seeds=seeds[i]) as net:
+            net.step(2 * window)
+            net.sync()
+            a = net.stats().events
+            t2 = time.perf_counter()
+            net.step(steps)
+            net.sync()
+            dt_p += time.perf_counter() - t2
+            ev_p += net.stats().events - a
+            for pop in range(3):
+                for age in range(15):
+                    same = same and np.array_equal(net.spikes(pop, age), b.spikes(i, pop, age))
+    b.close()
+    return {"instances": n, "steps": steps, "window": window, "graphs": True,
+            "tiles_by_bucket": tiles, "setup_s": setup_s, "events": ev_b, "seconds": dt_b,
+            "value": ev_b / dt_b, "neurons_total": int(N.sum()),
+            "plain_sample": len(sample), "plain_events": ev_p, "plain_seconds": dt_p,
+            "plain_value": ev_p / dt_p, "speedup_vs_plain": (ev_b / dt_b) / (ev_p / dt_p),
+            "sample_histories_identical": bool(same), "steps_before_timing": total_steps - steps,
+            "sweep": "N log-uniform [5000, 100000], p = min(0.1, 2000/N), rate U[10, 30] Hz, "
+                     "seeds {1337, i}, numpy PCG64(2024)"}
+
+
 def run_mode(mode: int, steps: int, warmup: int, tile: int | None = None):
     import spice2_amd as S
 
@@ -188,6 +255,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=300)
     ap.add_argument("--tile", type=int, default=None, help="targets per pull tile")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-sweep leg")
+    ap.add_argument("--batch-instances", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=1000)
+    ap.add_argument("--batch-window", type=int, default=100)
+    ap.add_argument("--plain-sample", type=int, default=16)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the known-size copy kernel (PMC counter calibration)")
     ap.add_argument("--traffic-summary", nargs=5, metavar=("FETCH_RUN", "WRITE_RUN", "CALIB_FETCH",
@@ -222,6 +294,9 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(net, 100, args.cpu_steps)
     net.close()
+    batch = None
+    if not args.no_batch:
+        batch = run_batch(args.batch_instances, args.batch_steps, args.batch_window, args.plain_sample)
 
     line = {
         "metric": "synaptic events/s (Brunel N=500k, p=0.02, deterministic rasters)",
@@ -252,6 +327,7 @@ def main():
                      "algorithmic_bytes_per_launch": alg_bytes / launches,
                      "avg_launch_ms": avg_ms},
         "cpu_baseline": cpu,
+        "batched": batch,
     }
     print(json.dumps(line), flush=True)
 

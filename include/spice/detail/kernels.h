@@ -117,12 +117,28 @@ __global__ void replay_address_kernel(replay_fn* out) {
 	*out = &replay_thunk<Syn, N>;
 }
 
+// Accesses through global-address-space pointers.  Generic (flat) accesses also
+// count against the counter that scalar and LDS loads wait on, so each such wait
+// would drain every load in flight (e.g. the prefetched event counts).
+__device__ inline uint32_t gload32(uint32_t const* p) {
+	using gptr = __attribute__((address_space(1))) uint32_t const*;
+	return *(gptr)(p);
+}
+__device__ inline void gstore32(uint32_t* p, uint32_t v) {
+	using gptr = __attribute__((address_space(1))) uint32_t*;
+	*(gptr)(p) = v;
+}
+__device__ inline void gstore64(uint64_t* p, uint64_t v) {
+	using gptr = __attribute__((address_space(1))) uint64_t*;
+	*(gptr)(p) = v;
+}
+
 template <class N>
 __device__ inline void load_state(N& n, uint32_t const* state, int64_t size, int64_t i) {
 	constexpr int W = int((sizeof(N) + 3) / 4);
 	uint32_t w[W];
 #pragma unroll
-	for (int k = 0; k < W; k++) w[k] = state[k * size + i];
+	for (int k = 0; k < W; k++) w[k] = gload32(state + k * size + i);
 	__builtin_memcpy(&n, w, sizeof(N));
 }
 
@@ -132,12 +148,12 @@ __device__ inline void store_state(N const& n, uint32_t* state, int64_t size, in
 	uint32_t w[W] = {};
 	__builtin_memcpy(w, &n, sizeof(N));
 #pragma unroll
-	for (int k = 0; k < W; k++) state[k * size + i] = w[k];
+	for (int k = 0; k < W; k++) gstore32(state + k * size + i, w[k]);
 }
 
-__device__ inline spice_step const& step_of(spice_block const& b, int32_t inst, int u) {
-	return b.win->steps[int64_t(inst) * SPICE_MAX_WINDOW + b.rel + u];
-}
+// The step table is written only by host copies between launches, so it is read
+// through the constant address space (scalar loads: the address is uniform).
+__device__ inline spice_step step_of(spice_block const& b, int32_t inst, int u);
 
 // Event counts of neuron i for application step `local`, all incoming
 // projections at once (independent loads, issued together).  MaxIn bounds the
@@ -150,11 +166,51 @@ __device__ inline void load_counts(uint32_t (&k)[MaxIn], spice_pop_view const& v
 		k[c] = 0;
 		if (c < v.nin) {
 			uint32_t* cp = v.in[c].counts + (local % v.in[c].slots) * v.size + i;
-			k[c]         = *cp;
-			if (clear && k[c]) *cp = 0;
+			k[c]         = gload32(cp);
+			if (clear && k[c]) gstore32(cp, 0);
 		}
 	}
 }
+
+// Reads a trivially copyable object that kernels never write (synapse functors,
+// replay slots) through the constant address space, word by word (device copies
+// are allocated in whole 16-byte units).
+template <class T>
+__device__ inline T cload_obj(void const* p) {
+	using cptr         = __attribute__((address_space(4))) uint32_t const*;
+	constexpr int W    = int((sizeof(T) + 3) / 4);
+	uint32_t w[W > 0 ? W : 1];
+#pragma unroll
+	for (int k = 0; k < W; k++) w[k] = ((cptr)(p))[k];
+	T t;
+	__builtin_memcpy(&t, w, sizeof(T));
+	return t;
+}
+
+__device__ inline spice_step step_of(spice_block const& b, int32_t inst, int u) {
+	spice_step const* steps = cload_obj<spice_window>(b.win).steps;
+	return cload_obj<spice_step>(steps + int64_t(inst) * SPICE_MAX_WINDOW + b.rel + u);
+}
+
+// The incoming projections of a population, loaded once per kernel: replay
+// slots and, for the kernel's own synapse type, a register copy of the functor.
+template <class Syn, int MaxIn>
+struct synapse_set {
+	replay_slot slot[MaxIn];
+	struct empty {};
+	std::conditional_t<std::is_void_v<Syn>, empty, Syn> syn[MaxIn];
+
+	__device__ void load(replay_slot const* slots, int nin) {
+#pragma unroll
+		for (int c = 0; c < MaxIn; c++) {
+			if (c < nin) {
+				slot[c] = cload_obj<replay_slot>(slots + c);
+				if constexpr (!std::is_void_v<Syn>)
+					if (slot[c].fn == nullptr) syn[c] = cload_obj<Syn>(slot[c].obj);
+			}
+		}
+	}
+};
 
 // deliver() k[c] times per incoming projection c, in connect order.  Without
 // Mixed every projection has the kernel's own synapse type and is inlined; with
@@ -162,15 +218,14 @@ __device__ inline void load_counts(uint32_t (&k)[MaxIn], spice_pop_view const& v
 // of the neuron, so that the neuron itself never has its address taken (it
 // stays in registers).
 template <class Syn, int MaxIn, bool Mixed, class N>
-__device__ inline void replay(N& n, uint32_t const (&k)[MaxIn], int nin, replay_slot const* slots) {
+__device__ inline void replay(N& n, uint32_t const (&k)[MaxIn], int nin, synapse_set<Syn, MaxIn> const& ss) {
 #pragma unroll
 	for (int c = 0; c < MaxIn; c++) {
 		if (c < nin && k[c] != 0) {
-			replay_slot const s = slots[c];
-			bool done           = false;
+			bool done = false;
 			if constexpr (!std::is_void_v<Syn>) {
-				if (!Mixed || s.fn == nullptr) {
-					Syn const& syn = *static_cast<Syn const*>(s.obj);
+				if (!Mixed || ss.slot[c].fn == nullptr) {
+					Syn const& syn = ss.syn[c];
 					for (uint32_t r = 0; r < k[c]; r++) syn.deliver(n);
 					done = true;
 				}
@@ -178,7 +233,7 @@ __device__ inline void replay(N& n, uint32_t const (&k)[MaxIn], int nin, replay_
 			if constexpr (Mixed || std::is_void_v<Syn>) {
 				if (!done) {
 					N tmp = n;
-					s.fn(s.obj, &tmp, k[c]);
+					ss.slot[c].fn(ss.slot[c].obj, &tmp, k[c]);
 					n = tmp;
 				}
 			}
@@ -203,7 +258,8 @@ __global__ void __launch_bounds__(256) update_lanes_kernel(spice_block b, group_
 	group_entry const e     = g.wg[blockIdx.x];
 	spice_pop_view const v  = b.pops[e.pop];  // a local copy: stores cannot alias it
 	Neur const neur         = g.neurs[e.member];
-	replay_slot const* sl   = g.slots[e.member];
+	synapse_set<Syn, MaxIn> ss;
+	if constexpr (!std::is_void_v<typename neuron_traits<Neur>::type>) ss.load(g.slots[e.member], v.nin);
 	int64_t const i         = e.first + threadIdx.x;
 	bool const active       = i < v.size;
 	int const lane          = threadIdx.x % 64;
@@ -213,21 +269,21 @@ __global__ void __launch_bounds__(256) update_lanes_kernel(spice_block b, group_
 		if (active) load_state(n, v.state, v.size, i);
 	uint32_t k[MaxIn] = {}, kn[MaxIn] = {};
 	if constexpr (!std::is_void_v<N>) {
-		spice_step const& s0 = step_of(b, v.inst, 0);
+		spice_step const s0 = step_of(b, v.inst, 0);
 		if (active && s0.local >= 0) load_counts<MaxIn>(k, v, s0.local, i, clear);
 	}
 	for (int u = 0; u < b.nsteps; u++) {
-		spice_step const& st = step_of(b, v.inst, u);
+		spice_step const st = step_of(b, v.inst, u);
 		if (st.local < 0) break;  // an instance's steps in a window form a prefix
 		bool spiked = false;
 		if constexpr (!std::is_void_v<N>) {
 			if (u + 1 < b.nsteps) {
-				spice_step const& sn = step_of(b, v.inst, u + 1);
+				spice_step const sn = step_of(b, v.inst, u + 1);
 				if (active && sn.local >= 0) load_counts<MaxIn>(kn, v, sn.local, i, clear);
 			}
 		}
 		if (active) {
-			if constexpr (!std::is_void_v<N>) replay<Syn, MaxIn, Mixed>(n, k, v.nin, sl);
+			if constexpr (!std::is_void_v<N>) replay<Syn, MaxIn, Mixed>(n, k, v.nin, ss);
 			device_stream rng{st.seed.s0, st.seed.s1, 0};
 			if constexpr (std::is_void_v<N>)
 				spiked = call_update<Neur>(neur, nullptr, st.dt, rng);
@@ -236,7 +292,7 @@ __global__ void __launch_bounds__(256) update_lanes_kernel(spice_block b, group_
 			if (rng.count != 0) atomicExch(v.error, 1);
 		}
 		uint64_t const bits = __ballot(spiked);
-		if (lane == 0 && (i >> 6) < v.bitmap_stride) v.bitmap[u * v.bitmap_stride + (i >> 6)] = bits;
+		if (lane == 0 && (i >> 6) < v.bitmap_stride) gstore64(v.bitmap + u * v.bitmap_stride + (i >> 6), bits);
 		if constexpr (!std::is_void_v<N>) {
 #pragma unroll
 			for (int c = 0; c < MaxIn; c++) k[c] = kn[c];
@@ -256,14 +312,14 @@ __global__ void __launch_bounds__(256) update_stateless_kernel(spice_block b, gr
 	spice_pop_view const v  = b.pops[e.pop];  // a local copy: stores cannot alias it
 	int const u             = blockIdx.y;
 	if (u >= b.nsteps) return;
-	spice_step const& st = step_of(b, v.inst, u);
+	spice_step const st = step_of(b, v.inst, u);
 	if (st.local < 0) return;
 	int64_t const c  = e.first + threadIdx.x;
 	int64_t const i0 = c * 64;
 	if (i0 >= v.size) return;
 	Neur const neur = g.neurs[e.member];
 	device_stream rng{st.seed.s0, st.seed.s1, 0};
-	jump_stream(rng, v.chunk_jump[c * 4]);
+	jump_stream(rng, cload_obj<spice_rng_state>(v.chunk_jump + c * 4));
 	int const n   = int(v.size - i0 < 64 ? v.size - i0 : 64);
 	uint64_t bits = 0;
 	for (int t = 0; t < n; t++) {
@@ -271,7 +327,7 @@ __global__ void __launch_bounds__(256) update_stateless_kernel(spice_block b, gr
 		bits |= uint64_t(spiked) << t;
 	}
 	if (rng.count != n * v.draws) atomicExch(v.error, 1);
-	v.bitmap[u * v.bitmap_stride + c] = bits;
+	gstore64(v.bitmap + u * v.bitmap_stride + c, bits);
 }
 
 // One thread per chunk of 16 consecutive neurons (stateful neurons that draw:
@@ -284,7 +340,8 @@ __global__ void __launch_bounds__(256) update_chunks_kernel(spice_block b, group
 	group_entry const e     = g.wg[blockIdx.x];
 	spice_pop_view const v  = b.pops[e.pop];  // a local copy: stores cannot alias it
 	Neur const neur         = g.neurs[e.member];
-	replay_slot const* sl   = g.slots[e.member];
+	synapse_set<Syn, MaxIn> ss;
+	if constexpr (!std::is_void_v<N>) ss.load(g.slots[e.member], v.nin);
 	int64_t const chunk     = e.first + threadIdx.x;
 	int64_t const i0        = chunk * C;
 	bool const active       = i0 < v.size;
@@ -292,7 +349,7 @@ __global__ void __launch_bounds__(256) update_chunks_kernel(spice_block b, group
 	bool const clear        = v.clear_counts != 0;
 	spice_rng_state const jp = active ? v.chunk_jump[chunk] : spice_rng_state{1, 0};
 	for (int u = 0; u < b.nsteps; u++) {
-		spice_step const& st = step_of(b, v.inst, u);
+		spice_step const st = step_of(b, v.inst, u);
 		if (st.local < 0) break;
 		uint32_t bits = 0;
 		if (active) {
@@ -308,7 +365,7 @@ __global__ void __launch_bounds__(256) update_chunks_kernel(spice_block b, group
 					load_state(n, v.state, v.size, i);
 					uint32_t k[MaxIn];
 					load_counts<MaxIn>(k, v, st.local, i, clear);
-					replay<Syn, MaxIn, Mixed>(n, k, v.nin, sl);
+					replay<Syn, MaxIn, Mixed>(n, k, v.nin, ss);
 					spiked = call_update<Neur>(neur, &n, st.dt, rng);
 					store_state(n, v.state, v.size, i);
 				}
@@ -319,7 +376,7 @@ __global__ void __launch_bounds__(256) update_chunks_kernel(spice_block b, group
 		uint64_t word = uint64_t(bits) << (16 * (lane % 4));
 		word |= __shfl_xor(word, 1, 64);
 		word |= __shfl_xor(word, 2, 64);
-		if (active && lane % 4 == 0) v.bitmap[u * v.bitmap_stride + (i0 >> 6)] = word;
+		if (active && lane % 4 == 0) gstore64(v.bitmap + u * v.bitmap_stride + (i0 >> 6), word);
 	}
 }
 
@@ -336,9 +393,11 @@ apply_pending_kernel(spice_pop_view const* pops, int64_t pop, int64_t local, rep
 	if (i >= v.size) return;
 	N n;
 	load_state(n, v.state, v.size, i);
+	synapse_set<Syn, MaxIn> ss;
+	ss.load(slots, v.nin);
 	uint32_t k[MaxIn];
 	load_counts<MaxIn>(k, v, local, i, true);
-	replay<Syn, MaxIn, Mixed>(n, k, v.nin, slots);
+	replay<Syn, MaxIn, Mixed>(n, k, v.nin, ss);
 	store_state(n, v.state, v.size, i);
 }
 

@@ -322,8 +322,8 @@ public:
 	// replayed through a device function pointer.
 	template <class Syn>
 	void add_incoming(Syn const& syn) {
-		void* obj = nullptr;
-		check_hip(hipMalloc(&obj, sizeof(Syn)), "hipMalloc");
+		void* obj = nullptr;  // whole 16-byte units: kernels read it word by word
+		check_hip(hipMalloc(&obj, (sizeof(Syn) + 15) / 16 * 16), "hipMalloc");
 		check_hip(hipMemcpy(obj, &syn, sizeof(Syn), hipMemcpyHostToDevice), "hipMemcpy");
 		_syn_objs.emplace_back(obj, [](void* p) { (void)hipFree(p); });
 		replay_fn* d_fn = nullptr;
@@ -354,7 +354,7 @@ public:
 		if (_slots_dirty) {
 			std::vector<replay_slot> h;
 			for (auto& in : _in) h.push_back({in.obj, in.type == _primary ? nullptr : in.fn});
-			if (h.empty()) h.push_back({nullptr, nullptr});
+			h.resize(std::max<size_t>(h.size(), 1), replay_slot{nullptr, nullptr});
 			_slots_dev.assign(h);
 			_slots_dirty  = false;
 			_eng->regroup = true;

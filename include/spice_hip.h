@@ -131,6 +131,16 @@ int spice_ctx_set_delivery(spice_ctx* ctx, int mode);
 /* Targets per pull tile (multiple of 64, <= 65536; LDS use is 2 bytes per target). */
 int spice_ctx_set_tile(spice_ctx* ctx, int64_t tile);
 int64_t spice_ctx_tile(spice_ctx* ctx);
+/* Pull tile of one target population (0: the context's); exact for any tile. */
+int spice_pop_set_tile(spice_ctx* ctx, int64_t pop, int64_t tile);
+int64_t spice_pop_tile(spice_ctx* ctx, int64_t pop);
+/* Tile autotuner (pull delivery): target populations are bucketed by size
+ * (< 4096, < 16384, < 65536, larger); for each bucket in turn every candidate
+ * tile is timed by replaying the delivery of the last block `reps` times (pull
+ * delivery rewrites the same counts, so the simulation is unchanged), and the
+ * fastest is kept.  chosen[4] receives the tile per bucket (0: empty bucket). */
+int spice_ctx_autotune(spice_ctx* ctx, int64_t const* candidates, int32_t ncand, int32_t reps,
+                       int64_t* chosen);
 int spice_ctx_set_timing(spice_ctx* ctx, int enabled);
 void* spice_ctx_stream(spice_ctx* ctx); /* hipStream_t all work of the context runs on */
 int spice_ctx_sync(spice_ctx* ctx);     /* waits and reports device-side errors */

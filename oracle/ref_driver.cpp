@@ -10,6 +10,8 @@
 // product must reproduce.  Rates and weights are parameters so that the same
 // driver covers the scaled and swept Brunel configurations.
 
+#include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <exception>
@@ -119,6 +121,60 @@ struct excitatory {
 	float weight;
 	void deliver(coba::neuron& to) const { to.Gex += weight; }
 };
+// samples/brunel+.cpp:59-99
+struct plastic {
+	struct synapse {
+		float W     = 1e-4;
+		float Zpre  = 0;
+		float Zpost = 0;
+	};
+	void deliver(synapse const& syn, lif::neuron& to) const { to.V += syn.W; }
+	void update(synapse& syn, float const dt, bool const pre, bool const post) const {
+		float const TstdpInv = 1.0f / 0.02f;
+		float const dtInv    = 1.0f / dt;
+		syn.W = std::clamp(syn.W - pre * 0.0202f * syn.W * std::exp(-syn.Zpost * dtInv) +
+		                       post * 0.01f * (1.0f - syn.W) * std::exp(-syn.Zpre * dtInv),
+		                   0.0f, 0.0003f);
+		syn.Zpre += pre;
+		syn.Zpost += post;
+		syn.Zpre -= syn.Zpre * dt * TstdpInv;
+		syn.Zpost -= syn.Zpost * dt * TstdpInv;
+	}
+	void skip(synapse& syn, float const dt, Int const n) const {
+		float const TstdpInv = 1.0f / 0.02f;
+		syn.Zpre *= std::pow(1 - dt * TstdpInv, n);
+		syn.Zpost *= std::pow(1 - dt * TstdpInv, n);
+	}
+};
+
+// Pair STDP on exponential traces (tau 20 ms, A+ 0.01, A- 0.0105 relative to w0,
+// W clipped to [0, 2 w0]) in the same update()/skip() form.
+struct stdp_pair {
+	float w0;
+	struct synapse {
+		float W     = 0;
+		float Zpre  = 0;
+		float Zpost = 0;
+	};
+	void init(synapse& syn, Int, Int, auto&) const { syn.W = w0; }
+	void deliver(synapse const& syn, lif::neuron& to) const { to.V += syn.W; }
+	void update(synapse& syn, float const dt, bool const pre, bool const post) const {
+		float const tauInv = 1.0f / 0.02f;
+		float const wmax   = 2.0f * w0;
+		if (pre) syn.W = std::clamp(syn.W - 0.0105f * w0 * syn.Zpost, 0.0f, wmax);
+		if (post) syn.W = std::clamp(syn.W + 0.01f * w0 * syn.Zpre, 0.0f, wmax);
+		syn.Zpre += pre;
+		syn.Zpost += post;
+		syn.Zpre -= syn.Zpre * dt * tauInv;
+		syn.Zpost -= syn.Zpost * dt * tauInv;
+	}
+	void skip(synapse& syn, float const dt, Int const n) const {
+		float const tauInv = 1.0f / 0.02f;
+		syn.Zpre *= std::pow(1 - dt * tauInv, n);
+		syn.Zpost *= std::pow(1 - dt * tauInv, n);
+	}
+};
+
 struct inhibitory {
 	float weight;
 	void deliver(coba::neuron& to) const { to.Gin += weight; }
@@ -312,6 +368,52 @@ int ref_brunel_run(int64_t nP, int64_t nE, int64_t nI, double p, float rate, flo
 			int64_t k = 0;
 			for (auto& n : E->get_neurons()) final_V[k++] = n.V;
 			for (auto& n : I->get_neurons()) final_V[k++] = n.V;
+		}
+		*raster_used = used;
+		return 0;
+	} catch (std::exception const& e) {
+		g_err = e.what();
+		return -1;
+	}
+}
+
+// Brunel+ (samples/brunel+.cpp:101-117): Brunel whose E->E projection is
+// plastic (rule 1: the sample's `plastic`, rule 2: stdp_pair with w0).  Same
+// outputs as ref_brunel_run.
+int ref_brunel_plastic_run(int64_t nP, int64_t nE, int64_t nI, double p, float rate, float wex, float win,
+                           float dt, float delay, uint32_t const* seeds, int nseeds, int rule, float w0,
+                           int64_t steps, int32_t* counts, int32_t* raster, int64_t raster_cap,
+                           int64_t* raster_used, uint64_t* raster_hash, uint64_t* state_hash) {
+	try {
+		snn net(dt, delay, make_seed(seeds, nseeds, 0));
+		auto P = net.add_population<poisson_r>(nP, {rate});
+		auto E = net.add_population<lif>(nE);
+		auto I = net.add_population<lif>(nI);
+		net.connect<fixed_weight>(P, E, fixed_probability(p), delay, {wex});
+		net.connect<fixed_weight>(P, I, fixed_probability(p), delay, {wex});
+		if (rule == 1)
+			net.connect<plastic>(E, E, fixed_probability(p), delay);
+		else
+			net.connect<stdp_pair>(E, E, fixed_probability(p), delay, {w0});
+		net.connect<fixed_weight>(E, I, fixed_probability(p), delay, {wex});
+		net.connect<fixed_weight>(I, E, fixed_probability(p), delay, {win});
+		net.connect<fixed_weight>(I, I, fixed_probability(p), delay, {win});
+		int64_t used = 0;
+		for (int64_t t = 0; t < steps; t++) {
+			net.step();
+			uint64_t rh = FNV_BASIS;
+			record(P, t, 0, counts, raster, raster_cap, used, rh);
+			record(E, t, 1, counts, raster, raster_cap, used, rh);
+			record(I, t, 2, counts, raster, raster_cap, used, rh);
+			if (raster_hash) raster_hash[t] = rh;
+			if (state_hash) {
+				uint64_t h = FNV_BASIS;
+				auto en    = E->get_neurons();
+				auto in    = I->get_neurons();
+				h          = fnv1a(h, en.data(), sizeof(lif::neuron) * en.size());
+				h          = fnv1a(h, in.data(), sizeof(lif::neuron) * in.size());
+				state_hash[t] = h;
+			}
 		}
 		*raster_used = used;
 		return 0;

@@ -222,6 +222,7 @@ typedef struct {
 	int32_t* ring;     /* max_delay * size */
 	int64_t* ring_len; /* max_delay */
 	int64_t nsteps;    /* steps recorded so far */
+	uint64_t* hist;    /* plastic populations: bit b = spiked b steps ago */
 } opop;
 
 typedef struct {
@@ -232,6 +233,10 @@ typedef struct {
 	int64_t *offsets;
 	int32_t* neighbors;
 	int owned;
+	int rule;          /* 0: fixed weight */
+	float w0;
+	float *W, *Zpre, *Zpost; /* per synapse, CSR order */
+	uint64_t* ages;    /* per source: (last touch + 1) | delivered << 63 */
 } oproj;
 
 struct orc_net {
@@ -275,12 +280,18 @@ void orc_net_destroy(orc_net* n) {
 		free(p->Twait);
 		free(p->ring);
 		free(p->ring_len);
+		free(p->hist);
 	}
-	for (int i = 0; i < n->nprojs; i++)
+	for (int i = 0; i < n->nprojs; i++) {
 		if (n->projs[i].owned) {
 			free(n->projs[i].offsets);
 			free(n->projs[i].neighbors);
 		}
+		free(n->projs[i].W);
+		free(n->projs[i].Zpre);
+		free(n->projs[i].Zpost);
+		free(n->projs[i].ages);
+	}
 	free(n);
 }
 
@@ -361,6 +372,176 @@ int orc_connect_csr(orc_net* n, int src, int dst, int64_t const* offsets, int32_
 	return n->nprojs++;
 }
 
+int orc_connect_plastic(orc_net* n, int src, int dst, double p, float delay, int rule, float w0) {
+	if (rule != ORC_RULE_BRUNELPLUS && rule != ORC_RULE_STDP_PAIR) return fail("unknown plastic rule");
+	int const c = orc_connect(n, src, dst, p, delay, 0.0f, ORC_V);
+	if (c < 0) return c;
+	oproj* pr       = &n->projs[c];
+	int64_t const S = n->pops[src].size;
+	int64_t const E = pr->offsets[S];
+	pr->rule        = rule;
+	pr->w0          = w0;
+	pr->W           = (float*)calloc(E ? E : 1, 4);
+	pr->Zpre        = (float*)calloc(E ? E : 1, 4);
+	pr->Zpost       = (float*)calloc(E ? E : 1, 4);
+	pr->ages        = (uint64_t*)calloc(S ? S : 1, 8); /* synapse_population.h:45 */
+	float const W0  = rule == ORC_RULE_BRUNELPLUS ? 1e-4f : w0;
+	for (int64_t e = 0; e < E; e++) pr->W[e] = W0;
+	if (rule == ORC_RULE_STDP_PAIR) (void)seed_take(n); /* PerSynapseInit: synapse_population.h:34-35 */
+	opop* sp = &n->pops[src];                           /* snn.h:46-47: source->plastic() */
+	if (!sp->hist) sp->hist = (uint64_t*)calloc(sp->size ? sp->size : 1, 8);
+	return c;
+}
+
+int orc_get_synapses(orc_net* n, int proj, float* W, float* Zpre, float* Zpost) {
+	oproj* pr = &n->projs[proj];
+	if (!pr->rule) return fail("not a plastic projection");
+	int64_t const E = pr->offsets[n->pops[pr->src].size];
+	memcpy(W, pr->W, 4 * (size_t)E);
+	memcpy(Zpre, pr->Zpre, 4 * (size_t)E);
+	memcpy(Zpost, pr->Zpost, 4 * (size_t)E);
+	return 0;
+}
+
+/* The rules' update() and skip() (brunel+.cpp:71-99 and the pair rule above). */
+static void syn_update(oproj const* pr, float* W, float* Zpre, float* Zpost, float dt, int pre, int post) {
+	if (pr->rule == ORC_RULE_COUNT) {
+		*W += 1.0f;
+	} else if (pr->rule == ORC_RULE_BRUNELPLUS) {
+		float const TstdpInv = 1.0f / 0.02f;
+		float const dtInv    = 1.0f / dt;
+		float const a  = (float)pre * 0.0202f * *W * expf(-*Zpost * dtInv);
+		float const b  = (float)post * 0.01f * (1.0f - *W) * expf(-*Zpre * dtInv);
+		float w        = *W - a + b;
+		*W             = w < 0.0f ? 0.0f : w > 0.0003f ? 0.0003f : w;
+		*Zpre += (float)pre;
+		*Zpost += (float)post;
+		*Zpre -= *Zpre * dt * TstdpInv;
+		*Zpost -= *Zpost * dt * TstdpInv;
+	} else {
+		float const tauInv = 1.0f / 0.02f;
+		float const wmax   = 2.0f * pr->w0;
+		if (pre) {
+			float const w = *W - 0.0105f * pr->w0 * *Zpost;
+			*W            = w < 0.0f ? 0.0f : w > wmax ? wmax : w;
+		}
+		if (post) {
+			float const w = *W + 0.01f * pr->w0 * *Zpre;
+			*W            = w < 0.0f ? 0.0f : w > wmax ? wmax : w;
+		}
+		*Zpre += (float)pre;
+		*Zpost += (float)post;
+		*Zpre -= *Zpre * dt * tauInv;
+		*Zpost -= *Zpost * dt * tauInv;
+	}
+}
+
+static void syn_skip(oproj const* pr, float* W, float* Zpre, float* Zpost, float dt, int64_t steps) {
+	if (pr->rule == ORC_RULE_COUNT) {
+		*W += (float)steps;
+		return;
+	}
+	float const tauInv = 1.0f / 0.02f;
+	double const f     = pow((double)(1 - dt * tauInv), (double)steps); /* std::pow(float, Int) */
+	*Zpre  = (float)((double)*Zpre * f);
+	*Zpost = (float)((double)*Zpost * f);
+}
+
+/* Brings synapse e of a spiking (or swept) source up to `time` with the target's
+ * spike history (synapse_population.h:97-115).  Shift counts are taken mod 64, as
+ * the reference's x86-64 code does for ~0 >> 64. */
+static int catch_up_hist(oproj* pr, uint64_t const* dst_hist, int64_t e, uint64_t ages, int64_t time,
+                         float dt) {
+	int const pre      = (int)(ages >> 63);
+	int64_t const age  = (int64_t)(ages & ~(1ull << 63));
+	if (!(time >= age)) return 0;
+	if (!dst_hist) return fail("Assertion failed (synapse_population.h:97): edge.first < dst_history.size()");
+	int64_t const prefix = 63 + pre - time + age;
+	uint64_t hist        = dst_hist[pr->neighbors[e]];
+	if (pre) syn_update(pr, &pr->W[e], &pr->Zpre[e], &pr->Zpost[e], dt, 1, (int)((hist >> (time - age)) & 1));
+	hist &= ~0ull >> (prefix & 63);
+	int64_t p = prefix;
+	while (hist) {
+		int64_t const lz = __builtin_clzll(hist);
+		syn_skip(pr, &pr->W[e], &pr->Zpre[e], &pr->Zpost[e], dt, lz - p);
+		syn_update(pr, &pr->W[e], &pr->Zpre[e], &pr->Zpost[e], dt, 0, 1);
+		hist ^= 1ull << (63 - lz);
+		p = lz + 1;
+	}
+	syn_skip(pr, &pr->W[e], &pr->Zpre[e], &pr->Zpost[e], dt, 64 - p);
+	return 0;
+}
+
+static int catch_up(orc_net* n, oproj* pr, int64_t e, uint64_t ages, int64_t time, float dt) {
+	return catch_up_hist(pr, n->pops[pr->dst].hist, e, ages, time, dt);
+}
+
+/* ---- one projection on its own (the reference's unit tests of the plastic
+ *      path, test/detail/synapse_population.cpp:84-167) ---------------------- */
+struct orc_syn {
+	oproj pr;
+	int64_t S;
+};
+
+orc_syn* orc_syn_create(int64_t S, int64_t D, int64_t const* offsets, int32_t const* neighbors, int rule,
+                        float w0) {
+	(void)D;
+	orc_syn* s      = (orc_syn*)calloc(1, sizeof(orc_syn));
+	int64_t const E = offsets[S];
+	s->S            = S;
+	s->pr.offsets   = (int64_t*)malloc(8 * (size_t)(S + 1));
+	s->pr.neighbors = (int32_t*)malloc(4 * (size_t)(E ? E : 1));
+	memcpy(s->pr.offsets, offsets, 8 * (size_t)(S + 1));
+	memcpy(s->pr.neighbors, neighbors, 4 * (size_t)E);
+	s->pr.rule  = rule;
+	s->pr.w0    = w0;
+	s->pr.W     = (float*)calloc(E ? E : 1, 4);
+	s->pr.Zpre  = (float*)calloc(E ? E : 1, 4);
+	s->pr.Zpost = (float*)calloc(E ? E : 1, 4);
+	s->pr.ages  = (uint64_t*)calloc(S ? S : 1, 8);
+	float const W0 = rule == ORC_RULE_BRUNELPLUS ? 1e-4f : rule == ORC_RULE_STDP_PAIR ? w0 : 0.0f;
+	for (int64_t e = 0; e < E; e++) s->pr.W[e] = W0;
+	return s;
+}
+
+void orc_syn_destroy(orc_syn* s) {
+	if (!s) return;
+	free(s->pr.offsets);
+	free(s->pr.neighbors);
+	free(s->pr.W);
+	free(s->pr.Zpre);
+	free(s->pr.Zpost);
+	free(s->pr.ages);
+	free(s);
+}
+
+int orc_syn_update(orc_syn* s, int64_t time, float dt, uint64_t const* dst_hist) {
+	for (int64_t src = 0; src < s->S; src++) {
+		for (int64_t e = s->pr.offsets[src]; e < s->pr.offsets[src + 1]; e++)
+			if (catch_up_hist(&s->pr, dst_hist, e, s->pr.ages[src], time, dt)) return -1;
+		s->pr.ages[src] = (uint64_t)(time + 1);
+	}
+	return 0;
+}
+
+/* deliver(syn, to): the count rule assigns the synapse's update count to the
+ * target (received_count = update_count), the others add W. */
+int orc_syn_deliver(orc_syn* s, int64_t time, float dt, int32_t const* spikes, int64_t nspikes,
+                    uint64_t const* dst_hist, float* dst) {
+	for (int64_t k = 0; k < nspikes; k++) {
+		int64_t const src = spikes[k];
+		for (int64_t e = s->pr.offsets[src]; e < s->pr.offsets[src + 1]; e++) {
+			if (catch_up_hist(&s->pr, dst_hist, e, s->pr.ages[src], time, dt)) return -1;
+			if (s->pr.rule == ORC_RULE_COUNT)
+				dst[s->pr.neighbors[e]] = s->pr.W[e];
+			else
+				dst[s->pr.neighbors[e]] += s->pr.W[e];
+		}
+		s->pr.ages[src] = (uint64_t)(time + 1) | (1ull << 63);
+	}
+	return 0;
+}
+
 static void pop_update(orc_net* n, opop* p, float dt, orc_rng* rng) {
 	int64_t const md = n->max_delay;
 	int64_t const slot = p->nsteps % md;
@@ -418,6 +599,10 @@ static void pop_update(orc_net* n, opop* p, float dt, orc_rng* rng) {
 			p->Gin[i] = Gin;
 		}
 	}
+	if (p->hist) { /* neuron_population.h:122-128 */
+		for (int64_t i = 0; i < p->size; i++) p->hist[i] <<= 1;
+		for (int64_t k = 0; k < cnt; k++) p->hist[out[k]] |= 1;
+	}
 	p->ring_len[slot] = cnt;
 	p->nsteps++;
 }
@@ -441,6 +626,16 @@ int orc_step(orc_net* n) { /* snn.cpp:7-28 */
 	orc_u128 s  = seed_take(n);
 	orc_rng rng = {s.lo, s.hi};
 	for (int i = 0; i < n->npops; i++) pop_update(n, &n->pops[i], dt, &rng);
+	if (n->time % 64 == 0) /* snn.cpp:17-19: catch every plastic synapse up (nominal dt) */
+		for (int c = 0; c < n->nprojs; c++) {
+			oproj* pr = &n->projs[c];
+			if (!pr->rule) continue;
+			for (int64_t src = 0; src < n->pops[pr->src].size; src++) {
+				for (int64_t e = pr->offsets[src]; e < pr->offsets[src + 1]; e++)
+					if (catch_up(n, pr, e, pr->ages[src], n->time, n->dt)) return -1;
+				pr->ages[src] = (uint64_t)(n->time + 1);
+			}
+		}
 	for (int c = 0; c < n->nprojs; c++) {
 		oproj* pr = &n->projs[c];
 		if (n->time < pr->delay - 1) continue;
@@ -449,8 +644,17 @@ int orc_step(orc_net* n) { /* snn.cpp:7-28 */
 		opop* dst         = &n->pops[pr->dst];
 		float* f = pr->field == ORC_V ? dst->V : pr->field == ORC_GEX ? dst->Gex : dst->Gin;
 		for (int64_t k = 0; k < cnt; k++) { /* synapse_population.h:88-134 */
-			int64_t const a = pr->offsets[ids[k]], b = pr->offsets[ids[k] + 1];
-			for (int64_t e = a; e < b; e++) f[pr->neighbors[e]] += pr->weight;
+			int64_t const src = ids[k];
+			int64_t const a = pr->offsets[src], b = pr->offsets[src + 1];
+			if (pr->rule) {
+				for (int64_t e = a; e < b; e++) {
+					if (catch_up(n, pr, e, pr->ages[src], n->time, n->dt)) return -1;
+					f[pr->neighbors[e]] += pr->W[e]; /* deliver(syn, to): to.V += syn.W */
+				}
+				pr->ages[src] = (uint64_t)(n->time + 1) | (1ull << 63);
+			} else {
+				for (int64_t e = a; e < b; e++) f[pr->neighbors[e]] += pr->weight;
+			}
 			n->events += b - a;
 		}
 	}

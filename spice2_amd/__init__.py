@@ -93,6 +93,10 @@ def runtime() -> C.CDLL:
         L.spice_pop_create_in.argtypes = [_vp, _i64, _i64, C.c_int32, C.c_int32, C.POINTER(_i64)]
         L.spice_ctx_set_graphs.argtypes = [_vp, C.c_int]
         L.spice_run.argtypes = [_vp, C.c_int32, _vp]
+        L.spice_pop_set_tile.argtypes = [_vp, _i64, _i64]
+        L.spice_pop_tile.restype = _i64
+        L.spice_pop_tile.argtypes = [_vp, _i64]
+        L.spice_ctx_autotune.argtypes = [_vp, C.POINTER(_i64), C.c_int32, C.c_int32, C.POINTER(_i64)]
         L.spice_rng_jump.argtypes = [_u64, _u64, _u64, C.POINTER(_u64), C.POINTER(_u64)]
         L.spice_rng_jump.restype = None
         L.spice_rng_jump_poly.argtypes = [_u64, C.POINTER(_u64), C.POINTER(_u64)]
@@ -200,6 +204,7 @@ def workloads() -> C.CDLL:
         L.wl_batch_spikes.argtypes = [_vp, C.c_int, C.c_int, _i64, _vp, _i64, C.POINTER(_i64)]
         L.wl_batch_context.restype = _vp
         L.wl_batch_context.argtypes = [_vp]
+        L.wl_batch_set_window.argtypes = [_vp, _i64]
         _wl = L
     return _wl
 
@@ -329,6 +334,17 @@ class BrunelBatch:
 
     def set_graphs(self, on: bool):
         self._check(self.L.wl_batch_set_graphs(self.h, int(on)))
+
+    def set_window(self, steps: int):
+        self._check(self.L.wl_batch_set_window(self.h, steps))
+
+    def autotune(self, candidates=(1024, 2048, 4096, 8192, 16384), reps: int = 3):
+        """Pull tile per target-size bucket (< 4096, < 16384, < 65536, larger),
+        timed by replaying the last block's delivery; returns the tiles chosen."""
+        c = (_i64 * len(candidates))(*candidates)
+        out = (_i64 * 4)()
+        check(runtime().spice_ctx_autotune(self.context(), c, len(candidates), reps, out))
+        return list(out)
 
     def sizes(self, inst: int):
         N = self.N[inst]

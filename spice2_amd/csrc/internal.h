@@ -35,6 +35,7 @@ struct Instance {
 
 struct Pop {
 	int32_t inst   = 0;
+	int64_t tile   = 0;  // pull tile of its incoming projections (0: the context's)
 	int64_t size   = 0;
 	int32_t words  = 0;
 	int32_t draws  = 0;
@@ -96,6 +97,7 @@ struct PullTile {
 };
 struct PullTarget {
 	int64_t inst;
+	int64_t tile;
 	int64_t dst_size;
 	int32_t nin;
 	int32_t proj[SPICE_MAX_IN];  // projection ids, connect order
@@ -125,7 +127,9 @@ struct spice_ctx {
 	spice_rt::PullTile* pull_tiles = nullptr;
 	spice_rt::PullTarget* pull_targets = nullptr;
 	int64_t pull_ntiles = 0;
-	int32_t pull_max_in = 0;
+	int32_t pull_pass = 1;  // projections a pull workgroup counts together
+	int64_t pull_max_tile = 64;
+	int32_t last_rel = -1, last_n = 0;  // last block run (autotuning replays its delivery)
 	int64_t tile = 8192;  // targets per pull tile
 	bool dev_dirty = true;
 	int32_t* error = nullptr;                 // device flag

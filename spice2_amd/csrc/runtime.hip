@@ -139,18 +139,24 @@ __global__ void __launch_bounds__(256) deliver_push_kernel(ProjDev const* projs,
 	}
 }
 
-// Pull delivery: one workgroup per (target tile, step).  For each incoming
-// projection in turn the workgroup counts, in LDS, the events of the spiking
-// sources that land in its tile and then writes the tile's counters once: no
-// global atomics.  Counters are 16 bits, two per LDS word; a target receives at
-// most one event per spiking source of a projection, so they cannot overflow
-// while fewer than 65536 sources spike - larger spike sets are processed in
-// chunks of 65535 whose counts are added into the (workgroup-owned) counters.
-// Each source's run of targets in the tile is read with 16-byte vector loads
-// (8 tile-local targets), four per lane in flight; a wave-wide scan of the run
-// lengths spreads the vectors of 64 sources evenly over the lanes.  Counts are
-// integers, so the result does not depend on the order of the LDS atomics and
-// equals the push result exactly.
+// Pull delivery: one workgroup per (target tile, step).  The workgroup counts,
+// in LDS, the events that the step's spikes of every incoming projection put
+// on each target of its tile, then writes the tile's counters once: no global
+// atomics.  Counters are 16 bits, two per LDS word; a target receives at most
+// one event per spiking source of a projection, so they cannot overflow while
+// fewer than 65536 sources of one projection spike - larger spike sets are
+// processed in chunks of 65535 whose counts are added into the (workgroup-owned)
+// global counters.  Counts are integers, so the result does not depend on the
+// order of the LDS atomics and equals the push result exactly.
+//
+// Work split: each spike's run of targets in the tile (a contiguous slice of its
+// CSR row, about p * tile targets) is read by a group of 8 lanes with 16-byte
+// vector loads (8 tile-local 16-bit targets each), so one load instruction of a
+// group reads one 128-byte line; every group keeps kSpikesInFlight runs x
+// kRunUnroll vectors per lane in flight.  The run descriptors (first vector,
+// length, projection) of a batch of spikes are computed first, one spike per
+// thread, and staged in LDS.
+//
 // Workgroup L handles tile L / 16 and step L % 16: the steps of one block land on
 // fixed XCDs (workgroups are dealt to the 8 XCDs round-robin), so the spike
 // lists and row data that all workgroups of a step read are cached in one L2.
@@ -162,106 +168,171 @@ struct PullArgs {
 	int32_t rel;
 	int32_t nsteps;
 	int64_t ring;
-	int64_t tile;
+	int64_t max_tile;  // largest tile of the launch (sets the LDS layout)
+	int32_t pass;      // projections counted together (<= kPassProj)
 	unsigned long long* events;
 };
 
-constexpr int kPullUnroll = 4;
+constexpr int kPullThreads    = 512;
+constexpr int kGroup          = 8;    // lanes per spike run
+constexpr int kRunUnroll      = 4;    // vectors per lane per run in flight
+constexpr int kSpikesInFlight = 2;    // runs per group in flight
+constexpr int kDescBatch      = 1024; // spike descriptors staged in LDS at once
+constexpr int kPassProj       = 4;    // projections whose counters share LDS in one pass
 
-__global__ void __launch_bounds__(512) deliver_pull_kernel(PullArgs a) {
-	extern __shared__ uint32_t lds_cnt[];  // tile / 2 words, two 16-bit counters each
+struct RunDesc {
+	uint32_t v0;    // first 16-byte vector of the run in btgt
+	uint32_t len;   // targets in the run
+	uint32_t meta;  // bits 0-2: offset of the run in its first vector; bits 3+: projection in pass
+};
+
+constexpr size_t kLdsMax = 160 * 1024;  // LDS of a CDNA4 compute unit
+
+// Loads through global-address-space pointers: generic (flat) loads also count
+// against the LDS wait counter, so every LDS wait would drain them as well.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+template <class T>
+__device__ inline T gload(T const* p) {
+	using gptr = __attribute__((address_space(1))) T const*;
+	return *(gptr)(p);
+}
+
+template <class T>
+__device__ inline void gstore(T* p, T v) {
+	using gptr = __attribute__((address_space(1))) T*;
+	*(gptr)(p) = v;
+}
+
+// LDS bytes of one pull workgroup counting `pass` projections of a tile together.
+size_t pull_lds_bytes(int64_t tile, int pass) {
+	return sizeof(uint32_t) * size_t(pass) * size_t((tile + 1) / 2) + sizeof(RunDesc) * kDescBatch;
+}
+
+// Projections per pass for a tile: as many as the incoming projections need,
+// at most kPassProj, within the LDS of a compute unit (0: the tile is too large).
+int pull_pass(int64_t tile, int max_in) {
+	int p = std::min(kPassProj, std::max(max_in, 1));
+	while (p > 0 && pull_lds_bytes(tile, p) > kLdsMax) p--;
+	return p;
+}
+
+__global__ void __launch_bounds__(kPullThreads) deliver_pull_kernel(PullArgs a) {
+	extern __shared__ uint32_t lds_cnt[];  // [kPassProj][words] two 16-bit counters per word, then descriptors
 	int const k = int(blockIdx.x % SPICE_MAX_BLOCK);
 	if (k >= a.nsteps) return;
 	PullTile const pt    = a.tiles[blockIdx.x / SPICE_MAX_BLOCK];
 	PullTarget const& tg = a.targets[pt.target];
 	int64_t const step   = local_step(a.win, tg.inst, a.rel + k);
 	if (step < 0) return;
-	int64_t const T0     = int64_t(pt.tau) * a.tile;
-	int64_t const Tn     = min<int64_t>(a.tile, tg.dst_size - T0);
-	int64_t const words  = (a.tile + 1) / 2;
-	int const tid = threadIdx.x, lane = tid % kWave, wave = tid / kWave, nwaves = blockDim.x / kWave;
-	unsigned long long ev = 0, rows = 0;
+	int64_t const T0    = int64_t(pt.tau) * tg.tile;
+	int64_t const Tn    = min<int64_t>(tg.tile, tg.dst_size - T0);
+	int64_t const words = (tg.tile + 1) / 2;
+	RunDesc* desc       = reinterpret_cast<RunDesc*>(lds_cnt + a.pass * ((a.max_tile + 1) / 2));
+	int const tid = threadIdx.x, lane = tid % kWave, group = tid / kGroup, gl = tid % kGroup;
+	constexpr int kGroups = kPullThreads / kGroup;
 	int64_t const slot_in = step % a.ring;
-	for (int j = 0; j < tg.nin; j++) {
-		ProjDev const& P   = a.projs[tg.proj[j]];
-		int64_t const nsp  = P.src_ring_cnt[slot_in];
-		int32_t const* ids = P.src_ring_ids + slot_in * P.src_size;
-		int64_t const rtw  = P.ntile + 1;
-		uint32_t* out      = P.counts + ((step + P.delay) % P.slots) * P.dst_size + T0;
-		if (pt.tau == 0 && tid == 0) rows += uint64_t(nsp);
-		for (int64_t c0 = 0; c0 == 0 || c0 < nsp; c0 += 65535) {
-			int64_t const c1 = min<int64_t>(nsp, c0 + 65535);
-			for (int64_t i = tid; i < words; i += blockDim.x) lds_cnt[i] = 0;
-			__syncthreads();
-			for (int64_t b0 = c0 + int64_t(wave) * kWave; b0 < c1; b0 += int64_t(nwaves) * kWave) {
-				int64_t const sp = b0 + lane;
-				uint64_t lo = 0, hi = 0;
-				if (sp < c1) {
-					int64_t const src  = ids[sp];
-					int64_t const base = P.offsets[src];
-					lo                 = uint64_t(base) + P.rt[src * rtw + pt.tau];
-					hi                 = uint64_t(base) + P.rt[src * rtw + pt.tau + 1];
+	unsigned long long ev = 0, rows = 0;
+
+	for (int j0 = 0; j0 < tg.nin; j0 += a.pass) {
+		int const np = min(a.pass, tg.nin - j0);
+		int64_t nsp[kPassProj];
+		int64_t most = 0;
+		for (int j = 0; j < kPassProj; j++) {
+			nsp[j] = j < np ? gload(a.projs[tg.proj[j0 + j]].src_ring_cnt + slot_in) : 0;
+			most   = max(most, nsp[j]);
+			if (pt.tau == 0 && tid == 0) rows += uint64_t(nsp[j]);
+		}
+		for (int64_t c0 = 0; c0 == 0 || c0 < most; c0 += 65535) {
+			// spikes of this chunk, projection by projection
+			int64_t pre[kPassProj + 1];
+			pre[0] = 0;
+			for (int j = 0; j < kPassProj; j++)
+				pre[j + 1] = pre[j] + max<int64_t>(0, min<int64_t>(nsp[j], c0 + 65535) - c0);
+			int64_t const M = pre[kPassProj];
+			for (int64_t i = tid; i < int64_t(np) * words; i += kPullThreads) lds_cnt[i] = 0;
+			for (int64_t d0 = 0; d0 < M; d0 += kDescBatch) {
+				int64_t const nd = min<int64_t>(kDescBatch, M - d0);
+				__syncthreads();  // counters cleared / previous descriptors consumed
+				for (int64_t f = tid; f < nd; f += kPullThreads) {
+					int64_t const g = d0 + f;
+					int j           = 0;
+					while (j + 1 < kPassProj && g >= pre[j + 1]) j++;
+					ProjDev const& P   = a.projs[tg.proj[j0 + j]];
+					int64_t const src  = gload(P.src_ring_ids + slot_in * P.src_size + c0 + (g - pre[j]));
+					int64_t const base = gload(P.offsets + src);
+					uint16_t const* rt = P.rt + src * (P.ntile + 1) + pt.tau;
+					int64_t const lo   = base + gload(rt);
+					int64_t const hi   = base + gload(rt + 1);
+					desc[f]            = {uint32_t(lo >> 3), uint32_t(hi - lo), uint32_t(lo & 7) | uint32_t(j << 3)};
+					ev += uint64_t(hi - lo);
 				}
-				ev += hi - lo;
-				uint64_t const v0 = lo >> 3;
-				uint32_t const nv = hi > lo ? uint32_t(((hi + 7) >> 3) - v0) : 0u;
-				uint32_t incl     = nv;
-				for (int off = 1; off < kWave; off <<= 1) {
-					uint32_t const t = __shfl_up(incl, off, kWave);
-					if (lane >= off) incl += t;
-				}
-				uint32_t const total = __shfl(incl, kWave - 1, kWave);
-				uint32_t const excl  = incl - nv;
-				// wave-uniform loop: every lane takes part in the shuffles (a
-				// shuffle from an inactive lane is undefined)
-				for (uint32_t vb = 0; vb < total; vb += kPullUnroll * kWave) {
-					uint64_t vq[kPullUnroll], lq[kPullUnroll], hq[kPullUnroll];
-					uint4 w[kPullUnroll];
+				__syncthreads();
+				for (int64_t r0 = group; r0 < nd; r0 += int64_t(kGroups) * kSpikesInFlight) {
+					RunDesc d[kSpikesInFlight];
+					uint16_t const* bt[kSpikesInFlight];
+					uint32_t* cnt[kSpikesInFlight];
+					uint32_t nv[kSpikesInFlight], most_nv = 0;
 #pragma unroll
-					for (int u = 0; u < kPullUnroll; u++) {
-						uint32_t const v = vb + u * kWave + lane;
-						int q            = 0;  // first lane whose inclusive count exceeds v
-						for (int s = 32; s > 0; s >>= 1)
-							if (__shfl(incl, q + s - 1, kWave) <= v) q += s;
-						q     = min(q, kWave - 1);
-						vq[u] = __shfl(v0, q, kWave) + (v - __shfl(excl, q, kWave));
-						lq[u] = __shfl(lo, q, kWave);
-						hq[u] = __shfl(hi, q, kWave);
+					for (int r = 0; r < kSpikesInFlight; r++) {
+						int64_t const idx = r0 + int64_t(r) * kGroups;
+						d[r]              = idx < nd ? desc[idx] : RunDesc{0, 0, 0};
+						int const j       = int(d[r].meta >> 3);
+						bt[r]             = a.projs[tg.proj[j0 + j]].btgt;
+						cnt[r]            = lds_cnt + j * words;
+						nv[r]             = d[r].len ? ((d[r].meta & 7) + d[r].len + 7) >> 3 : 0;
+						most_nv           = max(most_nv, nv[r]);
 					}
+					for (uint32_t vb = 0; vb < most_nv; vb += kGroup * kRunUnroll) {
+						u32x4 w[kSpikesInFlight][kRunUnroll];
 #pragma unroll
-					for (int u = 0; u < kPullUnroll; u++)
-						if (vb + u * kWave + lane < total)
-							w[u] = *reinterpret_cast<uint4 const*>(P.btgt + vq[u] * 8);
+						for (int r = 0; r < kSpikesInFlight; r++)
 #pragma unroll
-					for (int u = 0; u < kPullUnroll; u++) {
-						if (vb + u * kWave + lane >= total) continue;
-						uint32_t const wd[4] = {w[u].x, w[u].y, w[u].z, w[u].w};
-#pragma unroll
-						for (int e = 0; e < 8; e++) {
-							uint64_t const idx = vq[u] * 8 + e;
-							if (idx >= lq[u] && idx < hq[u]) {
-								uint32_t const t = (wd[e >> 1] >> (16 * (e & 1))) & 0xffffu;
-								atomicAdd(&lds_cnt[t >> 1], 1u << (16 * (t & 1)));
+							for (int u = 0; u < kRunUnroll; u++) {
+								uint32_t const vi = vb + uint32_t(u * kGroup + gl);
+								if (vi < nv[r])
+									w[r][u] = gload(reinterpret_cast<u32x4 const*>(bt[r] + (uint64_t(d[r].v0) + vi) * 8));
 							}
-						}
+#pragma unroll
+						for (int r = 0; r < kSpikesInFlight; r++)
+#pragma unroll
+							for (int u = 0; u < kRunUnroll; u++) {
+								uint32_t const vi = vb + uint32_t(u * kGroup + gl);
+								if (vi >= nv[r]) continue;
+								uint32_t const wd[4] = {w[r][u].x, w[r][u].y, w[r][u].z, w[r][u].w};
+								int32_t const first  = int32_t(vi * 8) - int32_t(d[r].meta & 7);
+#pragma unroll
+								for (int e = 0; e < 8; e++) {
+									int32_t const pos = first + e;  // position within the run
+									if (pos >= 0 && pos < int32_t(d[r].len)) {
+										uint32_t const t = (wd[e >> 1] >> (16 * (e & 1))) & 0xffffu;
+										atomicAdd(&cnt[r][t >> 1], 1u << (16 * (t & 1)));
+									}
+								}
+							}
 					}
 				}
 			}
 			__syncthreads();
-			for (int64_t i = tid; i < Tn; i += blockDim.x) {
-				uint32_t const c = (lds_cnt[i >> 1] >> (16 * (i & 1))) & 0xffffu;
-				if (c0 == 0)
-					out[i] = c;
-				else
-					out[i] += c;
+			for (int j = 0; j < np; j++) {
+				ProjDev const& P = a.projs[tg.proj[j0 + j]];
+				uint32_t* out    = P.counts + ((step + P.delay) % P.slots) * P.dst_size + T0;
+				uint32_t const* cj = lds_cnt + j * words;
+				for (int64_t i = tid; i < Tn; i += kPullThreads) {
+					uint32_t const c = (cj[i >> 1] >> (16 * (i & 1))) & 0xffffu;
+					gstore(out + i, c0 == 0 ? c : gload(out + i) + c);
+				}
 			}
 			__syncthreads();
 		}
 	}
+	(void)lane;
 	for (int off = 32; off > 0; off >>= 1) ev += __shfl_xor(ev, off, kWave);
 	if (lane == 0 && ev) atomicAdd(a.events, ev);
 	if (tid == 0 && rows) atomicAdd(a.events + 1, rows);
 }
+
+int64_t pop_tile(spice_ctx const* ctx, Pop const& P) { return P.tile ? P.tile : ctx->tile; }
 
 // Drops the cached window graphs (they hold the old descriptors and grids).
 void drop_graphs(spice_ctx* ctx) {
@@ -274,9 +345,11 @@ int refresh_device_descriptors(spice_ctx* ctx) {
 	drop_graphs(ctx);
 	SPICE_HIP(hipStreamSynchronize(ctx->stream));
 	if (ctx->mode == SPICE_DELIVER_PULL) {
-		for (auto& Q : ctx->projs)
-			if (Q.tile != ctx->tile)
-				if (int r = build_blocked(ctx, Q, ctx->tile)) return r;
+		for (auto& Q : ctx->projs) {
+			int64_t const t = pop_tile(ctx, ctx->pops[Q.dst]);
+			if (Q.tile != t)
+				if (int r = build_blocked(ctx, Q, t)) return r;
+		}
 	}
 	std::vector<PopDev> pd;
 	std::vector<spice_pop_view> views;
@@ -310,16 +383,31 @@ int refresh_device_descriptors(spice_ctx* ctx) {
 	// pull tiles: every population with incoming projections, split into tiles
 	std::vector<PullTarget> pt;
 	std::vector<PullTile> tiles;
+	int max_in        = 0;
+	ctx->pull_max_tile = 64;
+	for (auto& P : ctx->pops) {
+		max_in = std::max(max_in, int(P.incoming.size()));
+		if (!P.incoming.empty()) ctx->pull_max_tile = std::max(ctx->pull_max_tile, pop_tile(ctx, P));
+	}
+	ctx->pull_pass = pull_pass(ctx->pull_max_tile, max_in);
+	if (ctx->mode == SPICE_DELIVER_PULL && ctx->pull_pass == 0)
+		return set_error(SPICE_EINVAL, "pull tile too large for the LDS of a compute unit");
+	if (ctx->mode == SPICE_DELIVER_PULL) {
+		SPICE_HIP(hipFuncSetAttribute(reinterpret_cast<void const*>(&deliver_pull_kernel),
+		                              hipFuncAttributeMaxDynamicSharedMemorySize,
+		                              int(pull_lds_bytes(ctx->pull_max_tile, ctx->pull_pass))));
+	}
 	for (auto& P : ctx->pops) {
 		if (P.incoming.empty() || P.size == 0) continue;
 		PullTarget t{};
 		t.inst     = P.inst;
+		t.tile     = pop_tile(ctx, P);
 		t.dst_size = P.size;
 		t.nin      = int32_t(P.incoming.size());
 		for (size_t j = 0; j < P.incoming.size(); j++) t.proj[j] = int32_t(P.incoming[j]);
 		int32_t const id = int32_t(pt.size());
 		pt.push_back(t);
-		for (int64_t tau = 0; tau * ctx->tile < P.size; tau++) tiles.push_back({id, int32_t(tau)});
+		for (int64_t tau = 0; tau * t.tile < P.size; tau++) tiles.push_back({id, int32_t(tau)});
 	}
 	(void)hipFree(ctx->pull_tiles);
 	(void)hipFree(ctx->pull_targets);
@@ -386,6 +474,24 @@ int check_device_error(spice_ctx* ctx) {
 	return set_error(SPICE_EDEVICE, "device-side check failed (code " + std::to_string(flag) + ")");
 }
 
+// Delivers the spikes of steps rel .. rel+n-1 of the staged window.
+int launch_delivery(spice_ctx* ctx, int32_t rel, int32_t n) {
+	if (!ctx->projs.empty() && ctx->mode == SPICE_DELIVER_PUSH) {
+		unsigned const rowsy = unsigned(ctx->projs.size() * SPICE_MAX_BLOCK);
+		hipLaunchKernelGGL(deliver_push_kernel, dim3(kPushRowGroups, rowsy), dim3(256), 0, ctx->stream,
+		                   ctx->projs_dev, ctx->win_dev, rel, n, ctx->ring, ctx->events);
+		SPICE_HIP(hipGetLastError());
+	} else if (!ctx->projs.empty() && ctx->pull_ntiles > 0) {
+		PullArgs a{ctx->projs_dev, ctx->pull_tiles, ctx->pull_targets, ctx->win_dev, rel, n,
+		           ctx->ring, ctx->pull_max_tile, ctx->pull_pass, ctx->events};
+		hipLaunchKernelGGL(deliver_pull_kernel, dim3(unsigned(ctx->pull_ntiles * SPICE_MAX_BLOCK)),
+		                   dim3(kPullThreads), pull_lds_bytes(ctx->pull_max_tile, ctx->pull_pass),
+		                   ctx->stream, a);
+		SPICE_HIP(hipGetLastError());
+	}
+	return SPICE_OK;
+}
+
 // Launches the blocks of the staged window: per block the caller's neuron
 // updates, spike compaction and delivery.  Nothing here synchronises or
 // allocates, so a window can be captured into a graph.
@@ -409,19 +515,9 @@ int run_blocks(spice_ctx* ctx, int32_t nsteps, int32_t L, bool timed) {
 			SPICE_HIP(hipGetLastError());
 		}
 		if (timed) SPICE_HIP(hipEventRecord(tb.ev[2], ctx->stream));
-		if (!ctx->projs.empty() && ctx->mode == SPICE_DELIVER_PUSH) {
-			unsigned const rowsy = unsigned(ctx->projs.size() * SPICE_MAX_BLOCK);
-			hipLaunchKernelGGL(deliver_push_kernel, dim3(kPushRowGroups, rowsy), dim3(256), 0, ctx->stream,
-			                   ctx->projs_dev, ctx->win_dev, rel, n, ctx->ring, ctx->events);
-			SPICE_HIP(hipGetLastError());
-		} else if (!ctx->projs.empty() && ctx->pull_ntiles > 0) {
-			PullArgs a{ctx->projs_dev, ctx->pull_tiles, ctx->pull_targets, ctx->win_dev, rel, n,
-			           ctx->ring, ctx->tile, ctx->events};
-			size_t const lds = sizeof(uint32_t) * size_t((ctx->tile + 1) / 2);
-			hipLaunchKernelGGL(deliver_pull_kernel, dim3(unsigned(ctx->pull_ntiles * SPICE_MAX_BLOCK)), dim3(512),
-			                   lds, ctx->stream, a);
-			SPICE_HIP(hipGetLastError());
-		}
+		if (int r = launch_delivery(ctx, rel, n)) return r;
+		ctx->last_rel = rel;
+		ctx->last_n   = n;
 		if (timed) {
 			SPICE_HIP(hipEventRecord(tb.ev[3], ctx->stream));
 			ctx->timed.push_back(tb);
@@ -521,6 +617,22 @@ int spice_ctx_set_tile(spice_ctx* ctx, int64_t tile) {
 }
 
 int64_t spice_ctx_tile(spice_ctx* ctx) { return ctx->tile; }
+
+int spice_pop_set_tile(spice_ctx* ctx, int64_t pop, int64_t tile) {
+	if (pop < 0 || pop >= int64_t(ctx->pops.size())) return set_error(SPICE_EINVAL, "bad population");
+	if (tile != 0 && (tile < 64 || tile > 65536 || tile % 64))
+		return set_error(SPICE_EINVAL, "tile must be 0 or a multiple of 64 in [64, 65536]");
+	if (ctx->pops[pop].tile != tile) {
+		ctx->pops[pop].tile = tile;
+		ctx->dev_dirty      = true;
+	}
+	return SPICE_OK;
+}
+
+int64_t spice_pop_tile(spice_ctx* ctx, int64_t pop) {
+	if (pop < 0 || pop >= int64_t(ctx->pops.size())) return -1;
+	return pop_tile(ctx, ctx->pops[pop]);
+}
 
 int spice_ctx_set_timing(spice_ctx* ctx, int enabled) {
 	ctx->timing = enabled != 0;
@@ -769,6 +881,68 @@ int spice_ctx_set_update(spice_ctx* ctx, spice_update_fn fn, void* user) {
 int spice_ctx_set_graphs(spice_ctx* ctx, int enabled) {
 	ctx->graphs = enabled != 0;
 	if (!ctx->graphs) drop_graphs(ctx);
+	return SPICE_OK;
+}
+
+// Size bucket of a target population for tile tuning.
+static int tile_bucket(int64_t size) { return size < 4096 ? 0 : size < 16384 ? 1 : size < 65536 ? 2 : 3; }
+
+int spice_ctx_autotune(spice_ctx* ctx, int64_t const* candidates, int32_t ncand, int32_t reps,
+                       int64_t* chosen) {
+	if (ctx->mode != SPICE_DELIVER_PULL) return set_error(SPICE_EINVAL, "tile tuning needs pull delivery");
+	if (ctx->last_rel < 0) return set_error(SPICE_EINVAL, "tile tuning replays the last block: run steps first");
+	if (ncand < 1 || reps < 1) return set_error(SPICE_EINVAL, "no tile candidates");
+	for (int32_t c = 0; c < ncand; c++)
+		if (candidates[c] < 64 || candidates[c] > 65536 || candidates[c] % 64)
+			return set_error(SPICE_EINVAL, "tile candidates must be multiples of 64 in [64, 65536]");
+	SPICE_HIP(hipSetDevice(ctx->device));
+	// pull delivery overwrites the counts of the block it delivers, so replaying
+	// the last block is idempotent; only the event counters are restored
+	unsigned long long saved[2];
+	SPICE_HIP(hipStreamSynchronize(ctx->stream));
+	SPICE_HIP(hipMemcpy(saved, ctx->events, sizeof(saved), hipMemcpyDeviceToHost));
+	hipEvent_t e0, e1;
+	SPICE_HIP(hipEventCreate(&e0));
+	SPICE_HIP(hipEventCreate(&e1));
+	auto time_tiles = [&](float& ms) -> int {
+		if (int r = refresh_device_descriptors(ctx)) return r;
+		if (int r = launch_delivery(ctx, ctx->last_rel, ctx->last_n)) return r;  // warm
+		SPICE_HIP(hipEventRecord(e0, ctx->stream));
+		for (int32_t k = 0; k < reps; k++)
+			if (int r = launch_delivery(ctx, ctx->last_rel, ctx->last_n)) return r;
+		SPICE_HIP(hipEventRecord(e1, ctx->stream));
+		SPICE_HIP(hipEventSynchronize(e1));
+		SPICE_HIP(hipEventElapsedTime(&ms, e0, e1));
+		return SPICE_OK;
+	};
+	int r = SPICE_OK;
+	for (int b = 0; b < 4 && r == SPICE_OK; b++) {
+		chosen[b] = 0;
+		bool any  = false;
+		for (auto& P : ctx->pops) any = any || (!P.incoming.empty() && tile_bucket(P.size) == b);
+		if (!any) continue;
+		float best = 0;
+		for (int32_t c = 0; c < ncand && r == SPICE_OK; c++) {
+			for (auto& P : ctx->pops)
+				if (!P.incoming.empty() && tile_bucket(P.size) == b) P.tile = candidates[c];
+			ctx->dev_dirty = true;
+			float ms       = 0;
+			r              = time_tiles(ms);
+			if (r == SPICE_OK && (chosen[b] == 0 || ms < best)) {
+				best      = ms;
+				chosen[b] = candidates[c];
+			}
+		}
+		for (auto& P : ctx->pops)
+			if (!P.incoming.empty() && tile_bucket(P.size) == b) P.tile = chosen[b];
+		ctx->dev_dirty = true;
+	}
+	(void)hipEventDestroy(e0);
+	(void)hipEventDestroy(e1);
+	if (r) return r;
+	if (int r2 = refresh_device_descriptors(ctx)) return r2;
+	SPICE_HIP(hipStreamSynchronize(ctx->stream));
+	SPICE_HIP(hipMemcpy(ctx->events, saved, sizeof(saved), hipMemcpyHostToDevice));
 	return SPICE_OK;
 }
 

@@ -311,4 +311,13 @@ int wl_batch_spikes(void* h, int inst, int pop, int64_t age, int32_t* out, int64
 }
 
 spice_ctx* wl_batch_context(void* h) { return static_cast<brunel_batch*>(h)->b->context(); }
+
+int wl_batch_set_window(void* h, int64_t steps) {
+	try {
+		static_cast<brunel_batch*>(h)->b->set_window(steps);
+		return 0;
+	} catch (std::exception const& e) {
+		return fail(e);
+	}
+}
 }

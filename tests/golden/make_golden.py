@@ -154,10 +154,51 @@ def run_vogels(R, nE, nI, p, wex, win, delay, steps, seeds, init):
     return {"counts": cnt.reshape(steps, 3), "raster_hash": rh, "state_hash": sh}
 
 
+def run_plastic(R, N, p, rule, w0, steps, seeds):
+    """Brunel+ (samples/brunel+.cpp:101-117): E->E plastic with `rule`."""
+    nP, nE, nI = N // 2, N * 4 // 10, N // 10
+    s, n = seeds_arg(seeds)
+    cnt = np.zeros(steps * 3, np.int32)
+    used = np.zeros(1, np.int64)
+    rh = np.zeros(steps, np.uint64)
+    sh = np.zeros(steps, np.uint64)
+    r = R.ref_brunel_plastic_run(nP, nE, nI, p, 20.0, np.float32(2.0 / N), np.float32(-10.0 / N),
+                                 np.float32(1e-4), np.float32(15e-4), s, n, rule, np.float32(w0), steps,
+                                 P(cnt), None, 0, P(used), P(rh), P(sh))
+    assert r == 0, R.ref_last_error()
+    return {"counts": cnt.reshape(steps, 3), "raster_hash": rh, "state_hash": sh}
+
+
+PLASTIC_CASES = [  # name, N, p, rule, w0, steps
+    ("brunelplus_c0", 20000, 0.1, 1, 0.0, 300),
+    ("stdp_small", 4000, 0.1, 2, 2.0 / 4000, 600),
+]
+
+
+def plastic_fixtures(Rs, Rf):
+    """Adds the plastic-synapse fixtures to networks.json (other entries kept)."""
+    nets = json.loads((HERE / "networks.json").read_text())
+    for name, N, p, rule, w0, steps in PLASTIC_CASES:
+        b = run_plastic(Rs, N, p, rule, w0, steps, [1337])
+        a = run_plastic(Rf, N, p, rule, w0, steps, [1337])
+        np.savez_compressed(HERE / f"{name}.npz", counts=b["counts"], raster_hash=b["raster_hash"],
+                            state_hash=b["state_hash"], fast_counts=a["counts"])
+        nets[name] = {"N": N, "p": p, "rule": rule, "w0_bits": int(np.float32(w0).view(np.uint32)),
+                      "steps": steps, "seeds": [1337],
+                      "fast_equals_strict_raster": bool((a["raster_hash"] == b["raster_hash"]).all())}
+        print(name, b["counts"].sum(0), nets[name]["fast_equals_strict_raster"])
+    (HERE / "networks.json").write_text(json.dumps(nets, indent=1))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--big", action="store_true")
+    ap.add_argument("--plastic-only", action="store_true",
+                    help="only (re)write the plastic-synapse fixtures")
     args = ap.parse_args()
+    if args.plastic_only:
+        plastic_fixtures(O.ref(strict=True), O.ref(strict=False))
+        return
     Rf, Rs = O.ref(False), O.ref(True)
     t0 = time.time()
 

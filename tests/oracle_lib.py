@@ -86,6 +86,8 @@ def oracle() -> C.CDLL:
         L.orc_add_population.argtypes = [_vp, C.c_int, C.c_int64, C.c_float, C.c_int]
         L.orc_connect.argtypes = [_vp, C.c_int, C.c_int, C.c_double, C.c_float, C.c_float, C.c_int]
         L.orc_connect_csr.argtypes = [_vp, C.c_int, C.c_int, _vp, _vp, C.c_float, C.c_float, C.c_int]
+        L.orc_connect_plastic.argtypes = [_vp, C.c_int, C.c_int, C.c_double, C.c_float, C.c_int, C.c_float]
+        L.orc_get_synapses.argtypes = [_vp, C.c_int, _vp, _vp, _vp]
         L.orc_step.argtypes = [_vp]
         L.orc_spikes.restype = C.c_int64
         L.orc_spikes.argtypes = [_vp, C.c_int, C.c_int64, C.POINTER(C.POINTER(C.c_int32))]
@@ -176,8 +178,23 @@ class Net:
             raise ValueError(self.L.orc_last_error().decode())
         return r
 
+    def connect_plastic(self, src, dst, p, delay, rule, w0=0.0):
+        r = self.L.orc_connect_plastic(self.h, src, dst, p, delay, rule, w0)
+        if r < 0:
+            raise ValueError(self.L.orc_last_error().decode())
+        return r
+
+    def synapses(self, c):
+        """(W, Zpre, Zpost) of a plastic projection in CSR order."""
+        n = self.proj_synapses(c)
+        out = [np.zeros(max(n, 1), np.float32) for _ in range(3)]
+        if self.L.orc_get_synapses(self.h, c, *[_ptr(a) for a in out]) != 0:
+            raise ValueError(self.L.orc_last_error().decode())
+        return tuple(a[:n] for a in out)
+
     def step(self):
-        self.L.orc_step(self.h)
+        if self.L.orc_step(self.h) != 0:
+            raise ValueError(self.L.orc_last_error().decode())
 
     def spikes(self, pop, age=0):
         p = C.POINTER(C.c_int32)()
@@ -207,6 +224,26 @@ class Net:
 
     def proj_synapses(self, c):
         return int(self.L.orc_csr_synapses(self.h, c))
+
+
+ORC_RULE_BRUNELPLUS = 1
+ORC_RULE_STDP_PAIR = 2
+ORC_RULE_COUNT = 3
+
+
+def brunel_plastic_net(nP, nE, nI, p, rate, wex, win, rule, w0=0.0, dt=1e-4, delay=15e-4, seeds=(1337,)):
+    """Brunel+ (samples/brunel+.cpp:101-117): E->E plastic."""
+    n = Net(dt, delay, list(seeds))
+    P = n.add(ORC_POISSON, nP, rate)
+    E = n.add(ORC_LIF, nE)
+    I = n.add(ORC_LIF, nI)
+    n.connect(P, E, p, delay, wex)
+    n.connect(P, I, p, delay, wex)
+    n.connect_plastic(E, E, p, delay, rule, w0)
+    n.connect(E, I, p, delay, wex)
+    n.connect(I, E, p, delay, win)
+    n.connect(I, I, p, delay, win)
+    return n, (P, E, I)
 
 
 def brunel_net(nP, nE, nI, p, rate, wex, win, dt=1e-4, delay=15e-4, seeds=(1337,)):
@@ -255,5 +292,9 @@ def ref(strict: bool = False) -> C.CDLL:
         L.ref_vogels_run.argtypes = [C.c_int64, C.c_int64, C.c_double, C.c_float, C.c_float,
                                      C.c_float, C.c_float, C.c_int, _u32p, C.c_int, C.c_int64,
                                      _vp, _vp, C.c_int64, _vp, _vp, _vp]
+        L.ref_brunel_plastic_run.argtypes = [C.c_int64, C.c_int64, C.c_int64, C.c_double, C.c_float,
+                                             C.c_float, C.c_float, C.c_float, C.c_float, _u32p, C.c_int,
+                                             C.c_int, C.c_float, C.c_int64, _vp, _vp, C.c_int64, _vp,
+                                             _vp, _vp]
         _ref[strict] = L
     return _ref[strict]

@@ -78,3 +78,24 @@ def test_batch_graph_replay_equals_eager(mode):
                     for age in range(15):
                         assert np.array_equal(a.spikes(i, pop, age), g.spikes(i, pop, age)), (w, i, pop, age)
         assert a.stats().events == g.stats().events
+
+
+def test_autotuned_tiles_keep_rasters():
+    """Re-tiling target populations (the autotuner replays the last block's
+    delivery with each candidate tile) changes neither the counts it leaves
+    behind nor the rasters that follow."""
+    with make_batch() as a, make_batch() as t:
+        a.step(30)
+        t.step(30)
+        a.sync()
+        t.sync()
+        ev_before = t.stats().events
+        tiles = t.autotune((64, 1024, 4096), reps=1)
+        assert t.stats().events == ev_before
+        assert all(x in (0, 64, 1024, 4096) for x in tiles), tiles
+        for step in range(40):
+            a.step()
+            t.step()
+            for i in range(len(INSTANCES)):
+                for pop in range(3):
+                    assert np.array_equal(a.spikes(i, pop), t.spikes(i, pop)), (step, i, pop)

@@ -375,3 +375,71 @@ def test_random_brunel_networks_against_reference_build():
             net.step()
             assert [len(net.spikes(q)) for q in pops] == cnt.reshape(-1, 3)[t].tolist()
             assert net.state_hash() == int(sh[t])
+
+
+# ---------------------------------------------------------------- plastic path
+
+def _kat_syn():
+    """The projection of test/detail/synapse_population.cpp:30-44: 3 sources, 5
+    targets, edges (0,0) (0,1) (0,3) (1,3) (2,4)."""
+    off = np.array([0, 3, 4, 5], np.int64)
+    nb = np.array([0, 1, 3, 3, 4], np.int32)
+    L = O.oracle()
+    L.orc_syn_create.restype = O.C.c_void_p
+    L.orc_syn_create.argtypes = [O.C.c_int64, O.C.c_int64, O._vp, O._vp, O.C.c_int, O.C.c_float]
+    L.orc_syn_update.argtypes = [O._vp, O.C.c_int64, O.C.c_float, O._vp]
+    L.orc_syn_deliver.argtypes = [O._vp, O.C.c_int64, O.C.c_float, O._vp, O.C.c_int64, O._vp, O._vp]
+    L.orc_syn_destroy.argtypes = [O._vp]
+    return L, L.orc_syn_create(3, 5, O._ptr(off), O._ptr(nb), O.ORC_RULE_COUNT, 0.0)
+
+
+@pytest.mark.parametrize("ops,expected", [
+    ([("d", 0)], 1),                          # deliver
+    ([("u", 0), ("d", 0)], 1),                # update then deliver in one step: one update
+    ([("u", 0), ("u", 0), ("d", 0)], 1),      # repeated updates in one step: one update
+    ([("u", 0), ("d", 1)], 2),                # update and deliver in different steps: two
+    ([("d", 9)], 10),                         # skip ahead to step 10: ten updates
+    ([("u", 4), ("d", 9)], 10),
+])
+def test_plastic_catch_up_matches_reference_unit_tests(ops, expected):
+    """test/detail/synapse_population.cpp:96-167 (DeliverPlastic): after the
+    operations, targets 3 and 4 hold the update count of their synapses."""
+    L, s = _kat_syn()
+    hist = np.zeros(5, np.uint64)
+    dst = np.zeros(5, np.float32)
+    spikes = np.array([1, 2], np.int32)
+    try:
+        for op, t in ops:
+            if op == "u":
+                assert L.orc_syn_update(s, t, 1.0, O._ptr(hist)) == 0
+            else:
+                assert L.orc_syn_deliver(s, t, 1.0, O._ptr(spikes), 2, O._ptr(hist), O._ptr(dst)) == 0
+    finally:
+        L.orc_syn_destroy(s)
+    assert dst[3] == expected and dst[4] == expected
+
+
+@pytest.mark.parametrize("name", ["brunelplus_c0", "stdp_small"])
+def test_plastic_network_matches_reference(name):
+    """Brunel+ with the sample's plastic E->E synapse (N=20000, first 150 steps)
+    and with the pair-STDP rule (N=4000, 600 steps): spike counts, raster hashes
+    and neuron-state hashes of every step equal the reference's."""
+    rec = NETS[name]
+    z = np.load(G / f"{name}.npz")
+    N, p = rec["N"], rec["p"]
+    w0 = float(np.uint32(rec["w0_bits"]).view(np.float32))
+    steps = min(rec["steps"], 150 if N > 10000 else rec["steps"])
+    net, _ = O.brunel_plastic_net(N // 2, N * 4 // 10, N // 10, p, 20.0, np.float32(2.0 / N),
+                                  np.float32(-10.0 / N), rec["rule"], w0)
+    try:
+        for t in range(steps):
+            net.step()
+            lists = [net.spikes(k) for k in range(3)]
+            assert [len(x) for x in lists] == z["counts"][t].tolist(), t
+            h = 0xCBF29CE484222325
+            for x in lists:
+                h = int(O.oracle().orc_fnv1a(h, O._ptr(np.ascontiguousarray(x)), x.nbytes))
+            assert h == int(z["raster_hash"][t]), t
+            assert net.state_hash() == int(z["state_hash"][t]), t
+    finally:
+        net.close()
