@@ -31,10 +31,16 @@ namespace spice::detail {
 // deliver() k times through a type-erased pointer (targets with incoming
 // projections of several synapse types).
 using replay_fn = void (*)(void const* syn, void* neuron, uint32_t k);
+// deliver(synapse, neuron) for the events ev[0..k) of a projection with
+// per-synapse state, in ascending synapse id (= the reference's source order).
+using events_fn = void (*)(void const* syn, void* neuron, uint32_t const* ev, uint32_t k,
+                           uint32_t const* state, int64_t nnz);
 
 struct replay_slot {
 	void const* obj;  // device copy of the synapse functor
-	replay_fn fn;     // nullptr: the kernel's own synapse type, inlined
+	replay_fn fn;     // counted deliveries of another synapse type (else null)
+	events_fn ev_fn;  // event deliveries of another synapse type (else null)
+	// both null: the kernel's own synapse type, inlined
 };
 
 // One workgroup's share of a group launch: population `pop` (context id),
@@ -105,6 +111,46 @@ __device__ inline void jump_stream(device_stream& r, spice_rng_state c) {
 	r.count = 0;
 }
 
+template <class S>
+__device__ inline S load_synapse(uint32_t const* state, int64_t nnz, int64_t e) {
+	constexpr int W = int((sizeof(S) + 3) / 4);
+	uint32_t w[W];
+#pragma unroll
+	for (int k = 0; k < W; k++) w[k] = state[k * nnz + e];
+	S s;
+	__builtin_memcpy(&s, w, sizeof(S));
+	return s;
+}
+
+template <class S>
+__device__ inline void store_synapse(uint32_t* state, int64_t nnz, int64_t e, S const& s) {
+	constexpr int W = int((sizeof(S) + 3) / 4);
+	uint32_t w[W] = {};
+	__builtin_memcpy(w, &s, sizeof(S));
+#pragma unroll
+	for (int k = 0; k < W; k++) state[k * nnz + e] = w[k];
+}
+
+// The events of one target, applied by ascending synapse id: each round picks
+// the smallest id above the last one (lists are short: about p times the
+// source's spikes per step).
+template <class Syn, class N>
+__device__ inline void apply_events(N& n, Syn const& syn, uint32_t const* ev, uint32_t k, uint32_t const* state,
+                                    int64_t nnz) {
+	using S       = typename Syn::synapse;
+	uint32_t last = 0;
+	for (uint32_t j = 0; j < k; j++) {
+		uint32_t best = 0xffffffffu;
+		for (uint32_t q = 0; q < k; q++) {
+			uint32_t const e = ev[q];
+			if ((j == 0 || e > last) && e < best) best = e;
+		}
+		if (int64_t(best) >= nnz) break;  // lists always hold k valid ids; never read past the state
+		last = best;
+		syn.deliver(load_synapse<S>(state, nnz, best), n);
+	}
+}
+
 template <class Syn, class N>
 __device__ void replay_thunk(void const* syn, void* neuron, uint32_t k) {
 	Syn const& s = *static_cast<Syn const*>(syn);
@@ -113,8 +159,21 @@ __device__ void replay_thunk(void const* syn, void* neuron, uint32_t k) {
 }
 
 template <class Syn, class N>
-__global__ void replay_address_kernel(replay_fn* out) {
-	*out = &replay_thunk<Syn, N>;
+__device__ void events_thunk(void const* syn, void* neuron, uint32_t const* ev, uint32_t k, uint32_t const* state,
+                             int64_t nnz) {
+	apply_events<Syn>(*static_cast<N*>(neuron), *static_cast<Syn const*>(syn), ev, k, state, nnz);
+}
+
+// Device addresses of the thunks of a synapse type (one of them is null).
+template <class Syn, class N>
+__global__ void replay_address_kernel(replay_fn* out, events_fn* out_ev) {
+	if constexpr (StatefulSynapse<Syn>) {
+		*out    = nullptr;
+		*out_ev = &events_thunk<Syn, N>;
+	} else {
+		*out    = &replay_thunk<Syn, N>;
+		*out_ev = nullptr;
+	}
 }
 
 // Accesses through global-address-space pointers.  Generic (flat) accesses also
@@ -206,7 +265,7 @@ struct synapse_set {
 			if (c < nin) {
 				slot[c] = cload_obj<replay_slot>(slots + c);
 				if constexpr (!std::is_void_v<Syn>)
-					if (slot[c].fn == nullptr) syn[c] = cload_obj<Syn>(slot[c].obj);
+					if (slot[c].fn == nullptr && slot[c].ev_fn == nullptr) syn[c] = cload_obj<Syn>(slot[c].obj);
 			}
 		}
 	}
@@ -218,22 +277,34 @@ struct synapse_set {
 // of the neuron, so that the neuron itself never has its address taken (it
 // stays in registers).
 template <class Syn, int MaxIn, bool Mixed, class N>
-__device__ inline void replay(N& n, uint32_t const (&k)[MaxIn], int nin, synapse_set<Syn, MaxIn> const& ss) {
+__device__ inline void replay(N& n, uint32_t const (&k)[MaxIn], spice_pop_view const& v, int64_t i,
+                              synapse_set<Syn, MaxIn> const& ss) {
 #pragma unroll
 	for (int c = 0; c < MaxIn; c++) {
-		if (c < nin && k[c] != 0) {
-			bool done = false;
+		if (c < v.nin && k[c] != 0) {
+			replay_slot const& sl = ss.slot[c];
+			bool done             = false;
 			if constexpr (!std::is_void_v<Syn>) {
-				if (!Mixed || ss.slot[c].fn == nullptr) {
+				if (!Mixed || (sl.fn == nullptr && sl.ev_fn == nullptr)) {
 					Syn const& syn = ss.syn[c];
-					for (uint32_t r = 0; r < k[c]; r++) syn.deliver(n);
+					if constexpr (StatefulSynapse<Syn>) {
+						spice_in_view const& in = v.in[c];
+						apply_events<Syn>(n, syn, in.events + in.ev_offsets[i], k[c], in.syn_state, in.nnz);
+					} else {
+						for (uint32_t r = 0; r < k[c]; r++) syn.deliver(n);
+					}
 					done = true;
 				}
 			}
 			if constexpr (Mixed || std::is_void_v<Syn>) {
 				if (!done) {
 					N tmp = n;
-					ss.slot[c].fn(ss.slot[c].obj, &tmp, k[c]);
+					if (sl.ev_fn) {
+						spice_in_view const& in = v.in[c];
+						sl.ev_fn(sl.obj, &tmp, in.events + in.ev_offsets[i], k[c], in.syn_state, in.nnz);
+					} else {
+						sl.fn(sl.obj, &tmp, k[c]);
+					}
 					n = tmp;
 				}
 			}
@@ -283,7 +354,7 @@ __global__ void __launch_bounds__(256) update_lanes_kernel(spice_block b, group_
 			}
 		}
 		if (active) {
-			if constexpr (!std::is_void_v<N>) replay<Syn, MaxIn, Mixed>(n, k, v.nin, ss);
+			if constexpr (!std::is_void_v<N>) replay<Syn, MaxIn, Mixed>(n, k, v, i, ss);
 			device_stream rng{st.seed.s0, st.seed.s1, 0};
 			if constexpr (std::is_void_v<N>)
 				spiked = call_update<Neur>(neur, nullptr, st.dt, rng);
@@ -365,7 +436,7 @@ __global__ void __launch_bounds__(256) update_chunks_kernel(spice_block b, group
 					load_state(n, v.state, v.size, i);
 					uint32_t k[MaxIn];
 					load_counts<MaxIn>(k, v, st.local, i, clear);
-					replay<Syn, MaxIn, Mixed>(n, k, v.nin, ss);
+					replay<Syn, MaxIn, Mixed>(n, k, v, i, ss);
 					spiked = call_update<Neur>(neur, &n, st.dt, rng);
 					store_state(n, v.state, v.size, i);
 				}
@@ -397,8 +468,84 @@ apply_pending_kernel(spice_pop_view const* pops, int64_t pop, int64_t local, rep
 	ss.load(slots, v.nin);
 	uint32_t k[MaxIn];
 	load_counts<MaxIn>(k, v, local, i, true);
-	replay<Syn, MaxIn, Mixed>(n, k, v.nin, ss);
+	replay<Syn, MaxIn, Mixed>(n, k, v, i, ss);
 	store_state(n, v.state, v.size, i);
+}
+
+// ---------------------------------------------------------------- plastic synapses
+// The reference updates plastic synapses lazily (synapse_population.h:87-115):
+// a synapse is brought up to the current step only when its source's spike is
+// delivered or at the sweep every 64 steps (snn.cpp:17-19), replaying the
+// target's 64-step spike history with update() at the target's spikes and
+// skip() over the quiet stretches in between.  Same arithmetic here, per synapse
+// in parallel; shift counts are taken mod 64 as the reference's x86-64 code does.
+template <class Syn>
+__device__ inline void catch_up(Syn const& f, typename Syn::synapse& s, uint64_t ages, int64_t time, uint64_t hist,
+                                float dt) {
+	int const pre     = int(ages >> 63);
+	int64_t const age = int64_t(ages & ~(uint64_t(1) << 63));
+	if (!(time >= age)) return;
+	int64_t const prefix = 63 + pre - time + age;
+	if (pre) f.update(s, dt, true, ((hist >> (time - age)) & 1) != 0);
+	hist &= ~uint64_t(0) >> (prefix & 63);
+	int64_t p = prefix;
+	while (hist) {
+		int64_t const lz = __clzll(static_cast<long long>(hist));
+		f.skip(s, dt, Int(lz - p));
+		f.update(s, dt, false, true);
+		hist ^= uint64_t(1) << (63 - lz);
+		p = lz + 1;
+	}
+	f.skip(s, dt, Int(64 - p));
+}
+
+// Brings `row` (a source's synapses) up to `time`; one wave per row.
+template <class Syn>
+__device__ inline void catch_up_row(Syn const& f, spice_proj_view const& P, int64_t src, int64_t time, float dt,
+                                    uint64_t new_age, int lane) {
+	using S           = typename Syn::synapse;
+	uint64_t const ag = P.ages[src];
+	int64_t const a = P.offsets[src], b = P.offsets[src + 1];
+	for (int64_t e = a + lane; e < b; e += 64) {
+		S s = load_synapse<S>(P.syn_state, P.nnz, e);
+		catch_up(f, s, ag, time, P.dst_hist[P.neighbors[e]], dt);
+		store_synapse<S>(P.syn_state, P.nnz, e, s);
+	}
+	if (lane == 0) P.ages[src] = new_age;  // every lane has read the old value above
+}
+
+// The sweep at steps that are multiples of 64 (snn.cpp:17-19): every source.
+template <class Syn>
+__global__ void __launch_bounds__(256) plastic_sweep_kernel(spice_block b, int64_t proj, Syn const* fobj) {
+	spice_proj_view const P = b.projs[proj];
+	spice_step const st     = step_of(b, P.inst, 0);
+	if (st.local < 0 || st.local % 64 != 0) return;
+	Syn const f       = cload_obj<Syn>(fobj);
+	int const lane    = threadIdx.x % 64;
+	int64_t const gw  = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) / 64;
+	int64_t const nw  = int64_t(gridDim.x) * blockDim.x / 64;
+	for (int64_t src = gw; src < P.src_size; src += nw)
+		catch_up_row(f, P, src, st.local, st.dt_nominal, uint64_t(st.local + 1), lane);
+}
+
+// The plastic deliveries of the step (snn.cpp:21-25): the sources that spiked
+// delay - 1 steps ago are brought up to the step; their events were listed by
+// the runtime and are applied by the targets at the next step.
+template <class Syn>
+__global__ void __launch_bounds__(256) plastic_deliver_kernel(spice_block b, int64_t proj, Syn const* fobj) {
+	spice_proj_view const P = b.projs[proj];
+	spice_step const st     = step_of(b, P.inst, 0);
+	int64_t const t         = st.local;
+	if (t < 0 || t < P.delay - 1) return;
+	Syn const f          = cload_obj<Syn>(fobj);
+	int64_t const slot   = (t - P.delay + 1) % P.ring;
+	int64_t const nsp    = P.src_ring_cnt[slot];
+	int32_t const* ids   = P.src_ring_ids + slot * P.src_size;
+	int const lane       = threadIdx.x % 64;
+	int64_t const gw     = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) / 64;
+	int64_t const nw     = int64_t(gridDim.x) * blockDim.x / 64;
+	uint64_t const aged  = uint64_t(t + 1) | (uint64_t(1) << 63);
+	for (int64_t k = gw; k < nsp; k += nw) catch_up_row(f, P, ids[k], t, st.dt_nominal, aged, lane);
 }
 
 }  // namespace spice::detail

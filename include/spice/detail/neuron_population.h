@@ -74,6 +74,9 @@ struct engine {
 	Int window    = SPICE_MAX_WINDOW;  // steps per spice_run() (graphs are cached per length)
 	bool hold     = false;             // a batch queues whole rounds before flushing
 	std::string cb_error;
+	// per plastic projection (with its instance): its sweep and delivery kernels
+	// (synapse phase)
+	std::vector<std::pair<int64_t, std::function<void(spice_block const&, hipStream_t)>>> synapse_launchers;
 
 	explicit engine(Int max_delay) {
 		check_spice(spice_ctx_create(0, max_delay, &ctx));
@@ -92,10 +95,11 @@ struct engine {
 
 	hipStream_t stream() const { return static_cast<hipStream_t>(spice_ctx_stream(ctx)); }
 
-	void queue(int64_t inst, float dt, UInt128 seed) {
+	void queue(int64_t inst, float dt, float dt_nominal, UInt128 seed) {
 		spice_step s{};
-		s.dt   = dt;
-		s.seed = {seed.lo, seed.hi};
+		s.dt         = dt;
+		s.dt_nominal = dt_nominal;
+		s.seed       = {seed.lo, seed.hi};
 		pending[size_t(inst)].push_back(s);
 		if (!hold && Int(pending[size_t(inst)].size()) >= window) flush();
 	}
@@ -107,11 +111,17 @@ struct engine {
 	}
 
 	void flush();
+	// Drops an instance's queued steps, populations and plastic launchers and
+	// frees its device memory (a network that is destroyed).
+	void remove_instance(int64_t inst) noexcept;
 
 	static int launch_all(void* user, spice_block const* b, void* stream) {
 		auto* e = static_cast<engine*>(user);
 		try {
-			for (auto& g : e->groups) g.second->launch(*b, static_cast<hipStream_t>(stream));
+			if (b->phase == SPICE_PHASE_NEURONS)
+				for (auto& g : e->groups) g.second->launch(*b, static_cast<hipStream_t>(stream));
+			else
+				for (auto& l : e->synapse_launchers) l.second(*b, static_cast<hipStream_t>(stream));
 		} catch (std::exception const& x) {
 			e->cb_error = x.what();
 			return 1;
@@ -133,7 +143,17 @@ struct NeuronPopulation {
 	virtual std::type_index group_key() const              = 0;
 	virtual std::unique_ptr<group_base> make_group() const = 0;
 	virtual int64_t id() const                             = 0;
+	virtual int64_t instance() const                       = 0;
 };
+
+inline void engine::remove_instance(int64_t inst) noexcept {
+	if (size_t(inst) < pending.size()) pending[size_t(inst)].clear();
+	std::erase_if(pops, [&](NeuronPopulation* p) { return p->instance() == inst; });
+	std::erase_if(synapse_launchers, [&](auto const& l) { return l.first == inst; });
+	groups.clear();
+	regroup = true;
+	(void)spice_instance_destroy(ctx, inst);
+}
 
 inline void engine::flush() {
 	size_t const most = most_pending();
@@ -187,7 +207,8 @@ inline void engine::flush() {
 struct incoming {
 	std::type_index type;
 	void const* obj;   // device copy of the synapse functor
-	replay_fn fn;      // device address of its replay thunk
+	replay_fn fn;      // device address of its counted replay thunk (stateless synapses)
+	events_fn ev_fn;   // device address of its event thunk (stateful synapses)
 };
 
 template <Neuron Neur>
@@ -280,6 +301,7 @@ public:
 
 	Int size() const override { return _size; }
 	int64_t id() const override { return _id; }
+	int64_t instance() const override { return _inst; }
 	Neur const& functor() const { return _neuron; }
 	replay_slot const* slots_device() const { return _slots_dev.p; }
 
@@ -312,9 +334,9 @@ public:
 		return {_spike_buf.data(), size_t(cnt)};
 	}
 
-	void plastic() override {
-		throw std::logic_error("plastic synapses are not supported by the device engine");
-	}
+	// The reference keeps this population's spike history once it is the source
+	// of a plastic projection (snn.h:46-47); the runtime keeps it on the device.
+	void plastic() override { _plastic = true; }
 	std::span<UInt const> history() const override { return {}; }
 
 	// Registers an incoming projection (connect order).  The first synapse type
@@ -326,14 +348,18 @@ public:
 		check_hip(hipMalloc(&obj, (sizeof(Syn) + 15) / 16 * 16), "hipMalloc");
 		check_hip(hipMemcpy(obj, &syn, sizeof(Syn), hipMemcpyHostToDevice), "hipMemcpy");
 		_syn_objs.emplace_back(obj, [](void* p) { (void)hipFree(p); });
-		replay_fn* d_fn = nullptr;
-		check_hip(hipMalloc(&d_fn, sizeof(replay_fn)), "hipMalloc");
-		hipLaunchKernelGGL((replay_address_kernel<Syn, neuron_t>), dim3(1), dim3(1), 0, 0, d_fn);
+		void* d_fn = nullptr;  // the thunks' device addresses
+		check_hip(hipMalloc(&d_fn, sizeof(replay_fn) + sizeof(events_fn)), "hipMalloc");
+		auto* d_rf = static_cast<replay_fn*>(d_fn);
+		auto* d_ef = reinterpret_cast<events_fn*>(d_rf + 1);
+		hipLaunchKernelGGL((replay_address_kernel<Syn, neuron_t>), dim3(1), dim3(1), 0, 0, d_rf, d_ef);
 		check_hip(hipGetLastError(), "replay address");
-		replay_fn fn = nullptr;
-		check_hip(hipMemcpy(&fn, d_fn, sizeof(fn), hipMemcpyDeviceToHost), "hipMemcpy");
+		replay_fn fn  = nullptr;
+		events_fn efn = nullptr;
+		check_hip(hipMemcpy(&fn, d_rf, sizeof(fn), hipMemcpyDeviceToHost), "hipMemcpy");
+		check_hip(hipMemcpy(&efn, d_ef, sizeof(efn), hipMemcpyDeviceToHost), "hipMemcpy");
 		(void)hipFree(d_fn);
-		_in.push_back({std::type_index(typeid(Syn)), obj, fn});
+		_in.push_back({std::type_index(typeid(Syn)), obj, fn, efn});
 		if (_in.size() == 1) {
 			_primary  = std::type_index(typeid(Syn));
 			_reselect = [this] { set_group<Syn>(); };
@@ -353,8 +379,11 @@ public:
 		}
 		if (_slots_dirty) {
 			std::vector<replay_slot> h;
-			for (auto& in : _in) h.push_back({in.obj, in.type == _primary ? nullptr : in.fn});
-			h.resize(std::max<size_t>(h.size(), 1), replay_slot{nullptr, nullptr});
+			for (auto& in : _in) {
+				bool const own = in.type == _primary;
+				h.push_back({in.obj, own ? nullptr : in.fn, own ? nullptr : in.ev_fn});
+			}
+			h.resize(std::max<size_t>(h.size(), 1), replay_slot{nullptr, nullptr, nullptr});
 			_slots_dev.assign(h);
 			_slots_dirty  = false;
 			_eng->regroup = true;
@@ -385,6 +414,7 @@ private:
 	std::vector<std::conditional_t<stateful, neuron_t, char>> _host;
 	bool _host_valid = false, _host_dirty = false;
 	mutable std::vector<Int32> _spike_buf;
+	bool _plastic = false;
 	std::vector<incoming> _in;
 	std::vector<std::unique_ptr<void, void (*)(void*)>> _syn_objs;
 	std::type_index _primary   = std::type_index(typeid(void));

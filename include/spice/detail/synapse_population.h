@@ -25,33 +25,54 @@ struct SynapsePopulation {
 template <class Syn, Neuron SrcNeur, StatefulNeuron DstNeur>
 requires Synapse<Syn, SrcNeur, DstNeur>
 class synapse_population : public SynapsePopulation {
-	static_assert(!StatefulSynapse<Syn>,
-	              "synapses with per-synapse state are not supported by the device engine yet");
 	static_assert(DeliverTo<Syn, DstNeur>,
 	              "deliver(source, target) synapses are not supported by the device engine yet");
 	static_assert(std::is_trivially_copyable_v<Syn>, "synapse functors must be trivially copyable");
 
 public:
 	// Generates the connectivity (fixed_probability on the device, other
-	// topologies on the host) with `seed` and registers the projection with
-	// the target population (connect order = replay order).
-	synapse_population(engine* eng, Syn syn, Topology& c, util::seed_seq const& seed, Int const delay,
+	// topologies on the host) with the network's next seed, initialises the
+	// per-synapse state (with one more seed for a per-synapse init(),
+	// synapse_population.h:30-41) and registers the projection with the target
+	// population (connect order = replay order).
+	synapse_population(engine* eng, int64_t inst, Syn syn, Topology& c, util::seed_seq& seed, Int const delay,
 	                   neuron_population<SrcNeur>* src, neuron_population<DstNeur>* dst) :
 	_syn(std::move(syn)), _delay(delay) {
 		SPICE_PRE(delay >= 1);
-		UInt128 const s = seed.seed();
+		util::seed_seq const gs = seed++;  // the topology's seed (synapse_population.h:31)
+		UInt128 const s          = gs.seed();
 		if (auto* fp = dynamic_cast<fixed_probability*>(&c)) {
 			check_spice(spice_proj_create_fixed_probability(eng->ctx, src->id(), dst->id(), fp->probability(),
 			                                                s.lo, s.hi, delay, &_id));
 		} else {
 			std::vector<Int> offsets(size_t(c.src_count + 1), 0);
 			std::vector<Int32> neighbors(size_t(std::max<Int>(c.size(), 1)), 0);
-			c.generate(std::span<Int>(offsets), std::span<Int32>(neighbors.data(), size_t(c.size())), seed);
+			c.generate(std::span<Int>(offsets), std::span<Int32>(neighbors.data(), size_t(c.size())), gs);
 			check_spice(spice_proj_create_csr(eng->ctx, src->id(), dst->id(), offsets.data(),
 			                                  neighbors.data(), delay, &_id));
 		}
+		if constexpr (StatefulSynapse<Syn>) init_synapses(eng, c, seed);
+		if constexpr (PlasticSynapse<Syn>) {
+			check_spice(spice_proj_set_plastic(eng->ctx, _id));
+			src->plastic();
+			check_hip(hipMalloc(&_fobj, (sizeof(Syn) + 15) / 16 * 16), "hipMalloc");
+			check_hip(hipMemcpy(_fobj, &_syn, sizeof(Syn), hipMemcpyHostToDevice), "hipMemcpy");
+			int64_t const id   = _id;
+			Syn const* fobj    = static_cast<Syn const*>(_fobj);
+			int64_t const rows = src->size();
+			eng->synapse_launchers.emplace_back(inst, [id, fobj, rows](spice_block const& b, hipStream_t st) {
+				unsigned const sweep = unsigned(std::clamp<int64_t>((rows + 3) / 4, 1, 4096));
+				hipLaunchKernelGGL((plastic_sweep_kernel<Syn>), dim3(sweep), dim3(256), 0, st, b, id, fobj);
+				hipLaunchKernelGGL((plastic_deliver_kernel<Syn>), dim3(256), dim3(256), 0, st, b, id, fobj);
+				check_hip(hipGetLastError(), "plastic synapse launch");
+			});
+		}
 		dst->template add_incoming<Syn>(_syn);
 		eng->min_delay = std::min(eng->min_delay, delay);
+	}
+
+	~synapse_population() override {
+		if (_fobj) (void)hipFree(_fobj);
 	}
 
 	Int delay() const override { return _delay; }
@@ -60,6 +81,30 @@ public:
 private:
 	Syn _syn;
 	Int _delay;
-	int64_t _id = -1;
+	int64_t _id  = -1;
+	void* _fobj = nullptr;  // device copy of the functor (plastic kernels)
+
+	// Every synapse starts as a default synapse; a per-synapse init() runs on
+	// the host in CSR order with its own generator (synapse_population.h:34-40).
+	void init_synapses(engine* eng, Topology& c, util::seed_seq& seed) {
+		using S = typename Syn::synapse;
+		static_assert(std::is_trivially_copyable_v<S>, "synapse state must be trivially copyable");
+		if constexpr (PerSynapseInit<Syn>) {
+			util::xoroshiro64_128p rng(seed++);
+			int64_t nnz = 0;
+			check_spice(spice_proj_synapses(eng->ctx, _id, &nnz));
+			std::vector<Int> off(size_t(c.src_count + 1));
+			std::vector<Int32> nb(size_t(std::max<int64_t>(nnz, 1)));
+			check_spice(spice_proj_download(eng->ctx, _id, off.data(), nb.data()));
+			std::vector<S> syns(size_t(std::max<int64_t>(nnz, 1)));
+			for (Int src = 0; src < c.src_count; src++)
+				for (Int e = off[size_t(src)]; e < off[size_t(src) + 1]; e++)
+					_syn.init(syns[size_t(e)], src, nb[size_t(e)], rng);
+			check_spice(spice_proj_set_synapses(eng->ctx, _id, syns.data(), nnz, int32_t(sizeof(S))));
+		} else {
+			S const one{};
+			check_spice(spice_proj_set_synapses(eng->ctx, _id, &one, 1, int32_t(sizeof(S))));
+		}
+	}
 };
 }  // namespace spice::detail

@@ -1,0 +1,139 @@
+// Serving: a request loop that runs simulation jobs of mixed models and sizes
+// on one GPU with continuous batching (an extension; the reference runs one
+// network per process).
+//
+// Jobs arrive over time.  A job is admitted as soon as it has arrived and the
+// device memory it needs fits in the budget; it then becomes one instance of a
+// shared context (spice::batch) and steps together with every other admitted job
+// of its class, its state staying in device memory between windows.  A job that
+// has run its steps hands its network to the job's `finish` callback and is
+// freed, making room for the next.  Two classes run side by side on their own
+// streams: networks whose projections all have long delays (multi-step blocks)
+// and networks that need one-step blocks (delay-1 projections or per-synapse
+// state).  Each job's results are exactly those of the same network run alone:
+// instances never interact and every kernel is per instance.
+#pragma once
+
+#include <algorithm>
+#include <chrono>
+#include <functional>
+#include <memory>
+#include <thread>
+#include <utility>
+#include <vector>
+
+#include "spice/snn.h"
+
+namespace spice {
+
+struct request {
+	float dt        = 1e-4f;
+	float max_delay = 15e-4f;
+	util::seed_seq seed{{1337}};
+	Int steps = 0;
+	int64_t footprint = 0;     // device bytes the network holds (admission by memory)
+	bool single_step  = false; // needs one-step blocks (delay-1 projections or per-synapse state)
+	double arrival    = 0;     // seconds after run() starts
+	std::function<void(snn&)> build;
+	std::function<void(snn&)> finish;
+};
+
+class server {
+public:
+	struct record {
+		double arrival, admitted, finished;  // seconds after run() starts
+		Int steps;
+	};
+
+	// budget: device bytes the admitted jobs may hold at once; window: steps per
+	// round (a job's last round may be shorter); max_delay_steps bounds every
+	// job's max_delay / dt.
+	server(int64_t budget, Int window = 100, Int max_delay_steps = 64) :
+	_budget(budget), _window(window),
+	_multi(std::make_unique<batch>(max_delay_steps)), _single(std::make_unique<batch>(max_delay_steps)) {
+		_multi->set_window(window);
+		_single->set_window(window);
+	}
+
+	void submit(request r) { _queue.push_back(std::move(r)); }
+
+	// Runs until every submitted job has finished; returns one record per job in
+	// submission order.
+	std::vector<record> run() {
+		std::vector<record> out(_queue.size());
+		std::vector<size_t> order(_queue.size());  // jobs by arrival
+		for (size_t i = 0; i < order.size(); i++) order[i] = i;
+		std::stable_sort(order.begin(), order.end(),
+		                 [&](size_t a, size_t b) { return _queue[a].arrival < _queue[b].arrival; });
+		auto const t0 = std::chrono::steady_clock::now();
+		auto now      = [&] { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
+		size_t next   = 0;
+		int64_t used  = 0;
+		struct active {
+			size_t job;
+			snn* net;
+			batch* b;
+			Int left;
+		};
+		std::vector<active> act;
+		while (next < _queue.size() || !act.empty()) {
+			// admit every job that has arrived and fits
+			while (next < _queue.size() && _queue[order[next]].arrival <= now() &&
+			       (act.empty() || used + _queue[order[next]].footprint <= _budget)) {
+				request& r = _queue[order[next]];
+				batch* b   = r.single_step ? _single.get() : _multi.get();
+				snn& net   = b->add(r.dt, r.max_delay, r.seed);
+				r.build(net);
+				used += r.footprint;
+				out[order[next]] = {r.arrival, now(), 0.0, r.steps};
+				act.push_back({order[next], &net, b, r.steps});
+				next++;
+			}
+			if (act.empty()) {  // idle until the next arrival
+				std::this_thread::sleep_for(std::chrono::duration<double>(
+				    std::max(0.0, std::min(0.001, _queue[order[next]].arrival - now()))));
+				continue;
+			}
+			// one round: every admitted job queues up to `window` steps; both
+			// classes run their windows (asynchronously, on their own streams)
+			for (auto& a : act) {
+				Int const n = std::min(a.left, _window);
+				for (Int k = 0; k < n; k++) a.net->step();
+				a.left -= n;
+			}
+			_multi->flush();
+			_single->flush();
+			// retire finished jobs
+			for (size_t i = 0; i < act.size();) {
+				if (act[i].left > 0) {
+					i++;
+					continue;
+				}
+				request& r = _queue[act[i].job];
+				if (r.finish) r.finish(*act[i].net);
+				out[act[i].job].finished = now();
+				act[i].b->remove(*act[i].net);
+				used -= r.footprint;
+				act.erase(act.begin() + long(i));
+			}
+		}
+		_queue.clear();
+		return out;
+	}
+
+	// Synaptic events delivered by all jobs so far.
+	int64_t events() { return events(*_multi) + events(*_single); }
+
+private:
+	int64_t _budget;
+	Int _window;
+	std::unique_ptr<batch> _multi, _single;
+	std::vector<request> _queue;
+
+	static int64_t events(batch& b) {
+		spice_stats s{};
+		detail::check_spice(spice_ctx_stats(b.context(), &s));
+		return int64_t(s.events);
+	}
+};
+}  // namespace spice

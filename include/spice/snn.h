@@ -49,6 +49,12 @@ class snn {
 public:
 	snn(float const dt, float const max_delay, util::seed_seq seed) :
 	snn(nullptr, dt, max_delay, std::move(seed)) {}
+	snn(snn const&)            = delete;
+	snn& operator=(snn const&) = delete;
+	// Frees the network's device state; in a batch the other networks go on.
+	~snn() {
+		if (_eng) _eng->remove_instance(_inst);
+	}
 
 	template <Neuron Neur>
 	detail::neuron_population<Neur>* add_population(Int const size, Neur neur = {}) {
@@ -77,7 +83,7 @@ public:
 			throw std::logic_error("Assertion failed (topology.cpp:106): offsets.size() > src_count");
 		_eng->flush();  // steps queued so far ran without this projection
 		_synapses.push_back(std::make_unique<detail::synapse_population<Syn, SrcNeur, DstNeur>>(
-		    _eng.get(), std::move(syn), c, _seed++, d, source, target));
+		    _eng.get(), _inst, std::move(syn), c, _seed, d, source, target));
 	}
 
 	template <class Syn, Neuron SrcNeur, StatefulNeuron DstNeur>
@@ -92,7 +98,7 @@ public:
 		float const dt = _simtime += _dt;
 		if (_simtime >= 1) _simtime.reset();
 		UInt128 const s = (_seed++).seed();
-		_eng->queue(_inst, dt, s);
+		_eng->queue(_inst, dt, _dt, s);
 		_time++;
 	}
 
@@ -174,6 +180,12 @@ public:
 	void sync() {
 		_eng->flush();
 		detail::check_spice(spice_ctx_sync(_eng->ctx));
+	}
+	// Launches every queued step (asynchronously; results are read on demand).
+	void flush() { _eng->flush(); }
+	// Frees one network of the batch (its device memory is released; the others go on).
+	void remove(snn& net) {
+		std::erase_if(_nets, [&](std::unique_ptr<snn> const& p) { return p.get() == &net; });
 	}
 	void set_delivery(int mode) {
 		_eng->flush();

@@ -51,6 +51,13 @@ typedef struct {
 	uint32_t* counts; /* ring [slots][dst_size]: events for application step u at slot u % slots */
 	int64_t slots;
 	int64_t proj;
+	/* projections with per-synapse state: the next step's events of each target
+	 * are synapse ids events[ev_offsets[i] .. ev_offsets[i] + counts], in any
+	 * order (ascending synapse id = the reference's source order); null otherwise */
+	uint32_t const* ev_offsets;
+	uint32_t const* events;
+	uint32_t const* syn_state; /* SoA [syn_words][nnz] */
+	int64_t nnz;
 } spice_in_view;
 
 /* One population as the neuron-update kernels see it (device-resident, one per
@@ -71,12 +78,33 @@ typedef struct {
 	spice_in_view in[SPICE_MAX_IN]; /* connect order */
 	int32_t* error;         /* device flag: nonzero after a failed device-side check */
 	int32_t clear_counts;   /* the update kernel clears the counters it consumes (push mode) */
+	uint64_t* hist;         /* populations with plastic projections: bit b = spiked b steps ago */
 } spice_pop_view;
+
+/* One projection as the synapse kernels see it (device-resident, one per
+ * projection, indexed by projection id). */
+typedef struct {
+	int64_t const* offsets;      /* CSR [src_size + 1] */
+	int32_t const* neighbors;    /* CSR [nnz] */
+	int64_t nnz;
+	int64_t src_size, dst_size;
+	int64_t delay;
+	int64_t ring;                /* spike-ring slots of the source */
+	int32_t const* src_ring_ids; /* [ring][src_size] ascending */
+	int32_t const* src_ring_cnt; /* [ring] */
+	uint32_t* syn_state;         /* SoA [syn_words][nnz]; null without per-synapse state */
+	int32_t syn_words;
+	int32_t inst;
+	uint64_t* ages;              /* plastic: per source, (last touch + 1) | delivered << 63 */
+	uint64_t const* dst_hist;    /* plastic: target population's spike history */
+	uint32_t* ev_cursor;         /* [dst_size]: next free event slot of each target */
+	uint32_t* events;            /* [nnz] */
+} spice_proj_view;
 
 /* One step of one instance inside a window. */
 typedef struct {
 	float dt;                /* Kahan-compensated dt of the step (numeric.h:9-15) */
-	int32_t pad;
+	float dt_nominal;        /* the network's dt, which synapse updates receive (snn.cpp:18,22) */
 	int64_t local;           /* the instance's own step index; -1: the instance does not step */
 	spice_rng_state seed;    /* xoroshiro128+ seed of the step's stream (snn.cpp:12) */
 } spice_step;
@@ -90,19 +118,24 @@ typedef struct {
 	int32_t ninst;
 } spice_window;
 
-/* What a neuron-update kernel launch receives for one block of steps
+/* What the caller's kernel launches receive for one block of steps
  * rel .. rel+nsteps-1 of the running window (device pointers). */
 typedef struct {
 	spice_window const* win;
 	spice_pop_view const* pops;
+	spice_proj_view const* projs;
 	int32_t rel;
 	int32_t nsteps;
+	int32_t phase; /* SPICE_PHASE_NEURONS or SPICE_PHASE_SYNAPSES */
 } spice_block;
 
-/* Launches the neuron-update kernels of every population for one block, on
- * `stream` (a hipStream_t), without synchronising or allocating: blocks may be
- * captured into a HIP graph and replayed.  Returns 0, or nonzero after an error
- * (the callback keeps its own message). */
+#define SPICE_PHASE_NEURONS 0  /* neuron updates (first in a block) */
+#define SPICE_PHASE_SYNAPSES 1 /* plastic-synapse sweep and deliveries (last in a block) */
+
+/* Launches the caller's kernels of one phase of one block on `stream` (a
+ * hipStream_t), without synchronising or allocating: blocks may be captured
+ * into a HIP graph and replayed.  Returns 0, or nonzero after an error (the
+ * callback keeps its own message). */
 typedef int (*spice_update_fn)(void* user, spice_block const* block, void* stream);
 
 typedef struct {
@@ -177,6 +210,21 @@ int spice_proj_create_csr(spice_ctx* ctx, int64_t src_pop, int64_t dst_pop,
                           int64_t* out_proj);
 int spice_proj_synapses(spice_ctx* ctx, int64_t proj, int64_t* out_count);
 int spice_proj_download(spice_ctx* ctx, int64_t proj, int64_t* offsets, int32_t* neighbors);
+/* Per-synapse state (stateful synapses, synapse_population.h:30-41 and csr.h's
+ * edge payload): `count` AoS structs of `struct_bytes`, either one (every synapse
+ * starts from it) or one per synapse in CSR order.  The state lives SoA,
+ * [words][nnz].  The projection's deliveries then become per-target event lists
+ * that the target's update kernel applies in source order. */
+int spice_proj_set_synapses(spice_ctx* ctx, int64_t proj, void const* aos, int64_t count,
+                            int32_t struct_bytes);
+int spice_proj_get_synapses(spice_ctx* ctx, int64_t proj, void* aos, int32_t struct_bytes);
+/* Plastic synapses (update()/skip(), synapse_population.h:41-45,68-115): keeps
+ * per-source ages and the spike history of both populations, runs the
+ * SPICE_PHASE_SYNAPSES callback each step, and limits blocks to one step (a
+ * plastic synapse's weight at delivery depends on the target's spikes up to
+ * that step). */
+int spice_proj_set_plastic(spice_ctx* ctx, int64_t proj);
+int spice_ctx_proj_views(spice_ctx* ctx, spice_proj_view const** out);
 
 /* ---- instances (independent networks sharing one context: batched
  *      parameter sweeps and served jobs; no reference counterpart - the
@@ -189,6 +237,10 @@ int spice_proj_download(spice_ctx* ctx, int64_t proj, int64_t* offsets, int32_t*
 int spice_instance_create(spice_ctx* ctx, int64_t max_delay, int64_t* out_inst);
 int spice_pop_create_in(spice_ctx* ctx, int64_t inst, int64_t size, int32_t state_words,
                         int32_t draws, int64_t* out_pop);
+/* Frees an instance's populations and projections (served jobs that finished);
+ * their ids become invalid and the instance id is reused by a later
+ * spice_instance_create(). */
+int spice_instance_destroy(spice_ctx* ctx, int64_t inst);
 int64_t spice_instance_steps(spice_ctx* ctx, int64_t inst); /* steps the instance has run */
 int64_t spice_instance_count(spice_ctx* ctx);
 
@@ -198,10 +250,14 @@ int64_t spice_instance_count(spice_ctx* ctx);
  * whether it steps (an instance's active steps within a window form a prefix).
  * The window is split into blocks of at most min(SPICE_MAX_BLOCK, smallest
  * projection delay) steps; per block:
- *   1. the update callback launches every population's neuron-update kernel
- *      (each applies the deliveries due, updates, and sets spike bits);
+ *   1. the update callback (SPICE_PHASE_NEURONS) launches every population's
+ *      neuron-update kernel (each applies the deliveries due, updates, and
+ *      sets spike bits);
  *   2. the runtime compacts spike bits into ordered spike lists (the delay
- *      ring) and delivers them.
+ *      ring) and delivers them (event counts per target);
+ *   3. with plastic projections (one-step blocks), the runtime lays out each
+ *      target's event list and the callback (SPICE_PHASE_SYNAPSES) runs the
+ *      synapse sweep and the plastic deliveries.
  * The reference order of snn.cpp:7-28 holds exactly: a spike of step s is
  * delivered at step s+d-1 and applied before the update of step s+d, and a
  * block never spans a step that depends on a delivery made inside it.

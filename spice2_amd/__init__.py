@@ -185,6 +185,10 @@ def workloads() -> C.CDLL:
         L.wl_coba_create.restype = _vp
         L.wl_coba_create.argtypes = [_i64, _i64, C.c_double, C.c_float, C.c_float, C.c_float,
                                      C.c_float, C.c_int, C.POINTER(C.c_uint32), C.c_int, C.c_int]
+        L.wl_brunel_plastic_create.restype = _vp
+        L.wl_brunel_plastic_create.argtypes = [_i64, _i64, _i64, C.c_double, C.c_float, C.c_float, C.c_float,
+                                               C.c_float, C.c_float, C.POINTER(C.c_uint32), C.c_int, C.c_int,
+                                               C.c_float, C.c_int]
         L.wl_destroy.argtypes = [_vp]
         L.wl_step.argtypes = [_vp, _i64]
         L.wl_sync.argtypes = [_vp]
@@ -205,6 +209,11 @@ def workloads() -> C.CDLL:
         L.wl_batch_context.restype = _vp
         L.wl_batch_context.argtypes = [_vp]
         L.wl_batch_set_window.argtypes = [_vp, _i64]
+        _dp = C.POINTER(C.c_double)
+        _ip = C.POINTER(_i64)
+        L.wl_serve_run.argtypes = [C.c_int, C.POINTER(C.c_int), _ip, _dp, _ip, _dp, C.POINTER(C.c_uint32), _ip,
+                                   _i64, _i64, _dp, _dp, C.POINTER(_u64), _ip]
+        L.wl_job_alone.argtypes = [C.c_int, _i64, C.c_double, _i64, C.POINTER(C.c_uint32), C.POINTER(_u64)]
         _wl = L
     return _wl
 
@@ -229,6 +238,16 @@ class Network:
         L = workloads()
         s = (C.c_uint32 * len(seeds))(*seeds)
         h = L.wl_brunel_create(nP, nE, nI, p, rate, w_ex, w_in, dt, delay, s, len(seeds), mode)
+        return cls(h, (nP, nE, nI), 8)
+
+    @classmethod
+    def brunel_plastic(cls, nP, nE, nI, p, rate, w_ex, w_in, rule, w0=0.0, dt=1e-4, delay=15e-4,
+                       seeds=(1337,), mode=SPICE_DELIVER_PULL):
+        """Brunel+ : E->E plastic (rule 1: the sample's synapse, 2: pair STDP)."""
+        L = workloads()
+        s = (C.c_uint32 * len(seeds))(*seeds)
+        h = L.wl_brunel_plastic_create(nP, nE, nI, p, rate, w_ex, w_in, dt, delay, s, len(seeds), rule, w0,
+                                       mode)
         return cls(h, (nP, nE, nI), 8)
 
     @classmethod
@@ -286,6 +305,36 @@ class Network:
 
     def set_graphs(self, on: bool):
         self._check(self.L.wl_set_graphs(self.h, int(on)))
+
+
+def serve(kind, N, p, steps, arrival, seeds, footprint, budget, window=100):
+    """Runs jobs through spice::server (libspice_workloads.so, wl_serve_run).
+    Returns (admitted, finished, result hashes, synaptic events)."""
+    L = workloads()
+    n = len(kind)
+    arr = lambda t, xs: (t * n)(*xs)
+    adm = (C.c_double * n)()
+    fin = (C.c_double * n)()
+    res = (_u64 * n)()
+    ev = _i64()
+    sw = (C.c_uint32 * (2 * n))(*[int(w) for s in seeds for w in s])
+    r = L.wl_serve_run(n, arr(C.c_int, [int(x) for x in kind]), arr(_i64, [int(x) for x in N]),
+                       arr(C.c_double, [float(x) for x in p]), arr(_i64, [int(x) for x in steps]),
+                       arr(C.c_double, [float(x) for x in arrival]), sw, arr(_i64, [int(x) for x in footprint]),
+                       int(budget), int(window), adm, fin, res, C.byref(ev))
+    if r != 0:
+        raise SpiceError(L.wl_error().decode())
+    return np.array(adm[:]), np.array(fin[:]), [int(x) for x in res], ev.value
+
+
+def job_alone(kind, N, p, steps, seed):
+    """The result hash of one served job run on a network of its own."""
+    L = workloads()
+    out = _u64()
+    s = (C.c_uint32 * 2)(*seed)
+    if L.wl_job_alone(int(kind), int(N), float(p), int(steps), s, C.byref(out)) != 0:
+        raise SpiceError(L.wl_error().decode())
+    return out.value
 
 
 class BrunelBatch:

@@ -31,9 +31,11 @@ struct Instance {
 	int64_t max_delay = 1;
 	int64_t steps     = 0;  // steps run
 	uint64_t draws    = 0;  // stream outputs taken per step by its populations so far
+	bool live         = true;
 };
 
 struct Pop {
+	bool dead      = false;  // its instance was destroyed; the id is not reused
 	int32_t inst   = 0;
 	int64_t tile   = 0;  // pull tile of its incoming projections (0: the context's)
 	int64_t size   = 0;
@@ -45,10 +47,12 @@ struct Pop {
 	int32_t* ring_ids = nullptr;   // [ring][size]
 	int32_t* ring_cnt = nullptr;   // [ring]
 	xoro_state* chunk_jump = nullptr; // per 16 neurons: x^(stream offset in the instance) mod P
+	uint64_t* hist = nullptr;      // spike history (populations with plastic projections)
 	std::vector<int64_t> incoming; // projection ids, connect order
 };
 
 struct Proj {
+	bool dead = false;
 	int64_t src = 0, dst = 0;
 	int64_t delay = 1;
 	int64_t* offsets = nullptr;    // [src_size + 1]
@@ -62,6 +66,14 @@ struct Proj {
 	int64_t ntile = 0;
 	uint16_t* rt = nullptr;        // [src_size][ntile + 1]
 	uint16_t* btgt = nullptr;      // [nnz], padded to whole 16-byte vectors
+	// per-synapse state and event lists (stateful synapses)
+	int32_t syn_words = 0;
+	uint32_t* syn_state = nullptr; // [syn_words][nnz]
+	uint32_t* ev_offsets = nullptr;// [dst_size]
+	uint32_t* ev_cursor = nullptr; // [dst_size]
+	uint32_t* events = nullptr;    // [nnz]
+	bool plastic = false;
+	uint64_t* ages = nullptr;      // [src_size]
 };
 
 // Device-side descriptors, rebuilt when populations/projections change.
@@ -72,6 +84,7 @@ struct PopDev {
 	int64_t size;
 	int32_t* ring_ids;
 	int32_t* ring_cnt;
+	uint64_t* hist;
 };
 
 struct ProjDev {
@@ -88,6 +101,7 @@ struct ProjDev {
 	uint16_t const* rt;
 	uint16_t const* btgt;
 	int64_t ntile;
+	uint32_t* ev_offsets;
 };
 
 // Pull delivery: one workgroup per (target tile, step).
@@ -135,6 +149,11 @@ struct spice_ctx {
 	int32_t* error = nullptr;                 // device flag
 	unsigned long long* events = nullptr;     // device counters: events, rows
 	spice_pop_view* views_dev = nullptr;    // [pops]
+	spice_proj_view* proj_views_dev = nullptr; // [projs]
+	int64_t* stateful_dev = nullptr;        // ids of projections with per-synapse state
+	int64_t nstateful = 0;
+	bool any_plastic = false;
+	int64_t max_pop_size = 0;
 	// the running window: device table [insts][SPICE_MAX_WINDOW] filled from one
 	// of two pinned staging buffers (the other may still be in flight)
 	spice_window* win_dev = nullptr;

@@ -69,6 +69,7 @@ __global__ void __launch_bounds__(256)
 compact_kernel(PopDev const* pops, spice_window const* win, int rel, int64_t ring) {
 	PopDev const P  = pops[blockIdx.x];
 	int const k     = blockIdx.y;
+	if (P.inst < 0) return;  // destroyed
 	int64_t const t = local_step(win, P.inst, rel + k);
 	if (t < 0) return;
 	int64_t const nw = (P.size + 63) / 64;
@@ -104,6 +105,21 @@ compact_kernel(PopDev const* pops, spice_window const* win, int rel, int64_t rin
 	if (tid == 255) P.ring_cnt[slot] = scan[255];
 }
 
+// Spike history of populations with plastic projections (neuron_population.h:122-128):
+// per step of the block, hist = hist << 1 | spiked.
+__global__ void __launch_bounds__(256) history_kernel(PopDev const* pops, spice_window const* win, int rel, int n) {
+	PopDev const P = pops[blockIdx.y];
+	if (!P.hist || P.inst < 0) return;
+	int64_t const i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (i >= P.size) return;
+	uint64_t h = P.hist[i];
+	for (int u = 0; u < n; u++) {
+		if (local_step(win, P.inst, rel + u) < 0) break;
+		h = (h << 1) | ((P.bitmap[u * P.bitmap_stride + (i >> 6)] >> (i & 63)) & 1u);
+	}
+	P.hist[i] = h;
+}
+
 // Push delivery: one workgroup row per (projection, step); its waves walk the
 // step's spikes, each wave streaming one spike's CSR row and incrementing the
 // target's event counter for the application step with a device-scope atomic.
@@ -116,6 +132,7 @@ __global__ void __launch_bounds__(256) deliver_push_kernel(ProjDev const* projs,
 	int const k = int(blockIdx.y % SPICE_MAX_BLOCK);
 	if (k >= nsteps) return;
 	ProjDev const& P = projs[blockIdx.y / SPICE_MAX_BLOCK];
+	if (P.inst < 0) return;  // destroyed
 	int64_t const t  = local_step(win, P.inst, rel + k);
 	if (t < 0) return;
 	int64_t const slot = t % ring;
@@ -332,6 +349,68 @@ __global__ void __launch_bounds__(kPullThreads) deliver_pull_kernel(PullArgs a) 
 	if (tid == 0 && rows) atomicAdd(a.events + 1, rows);
 }
 
+// Event lists of projections with per-synapse state (one-step blocks): the
+// events a target receives at the next step are laid out contiguously, target
+// by target.  One workgroup per projection scans the targets' event counts of
+// the next step (written by the counting delivery) into list offsets.
+__global__ void __launch_bounds__(1024)
+event_offsets_kernel(spice_proj_view const* pv, int64_t const* ids, ProjDev const* projs, spice_window const* win,
+                     int rel) {
+	int64_t const c       = ids[blockIdx.x];
+	spice_proj_view const P = pv[c];
+	int64_t const t       = local_step(win, P.inst, rel);
+	if (t < 0) return;
+	ProjDev const& Q       = projs[c];
+	uint32_t const* counts = Q.counts + ((t + 1) % Q.slots) * P.dst_size;
+	int64_t const D        = P.dst_size;
+	int const tid          = threadIdx.x;
+	int64_t const per      = (D + blockDim.x - 1) / blockDim.x;
+	int64_t const a = min<int64_t>(D, tid * per), b = min<int64_t>(D, a + per);
+	uint32_t sum = 0;
+	for (int64_t i = a; i < b; i++) sum += counts[i];
+	__shared__ uint32_t scan[1024];
+	scan[tid] = sum;
+	__syncthreads();
+	for (int off = 1; off < 1024; off <<= 1) {
+		uint32_t const v = tid >= off ? scan[tid - off] : 0;
+		__syncthreads();
+		scan[tid] += v;
+		__syncthreads();
+	}
+	uint32_t pos = scan[tid] - sum;
+	uint32_t* offs = Q.ev_offsets;
+	for (int64_t i = a; i < b; i++) {
+		offs[i]        = pos;
+		P.ev_cursor[i] = pos;
+		pos += counts[i];
+	}
+}
+
+// Appends the synapse ids of the spikes delivered this step (spikes of step
+// t - delay + 1, snn.cpp:21-25) to their targets' event lists: one wave per
+// spiking source, lanes over its row.  The order inside a list is arbitrary;
+// the target's kernel applies them by ascending synapse id.
+__global__ void __launch_bounds__(256)
+event_scatter_kernel(spice_proj_view const* pv, int64_t const* ids, spice_window const* win, int rel) {
+	spice_proj_view const P = pv[ids[blockIdx.y]];
+	int64_t const t         = local_step(win, P.inst, rel);
+	if (t < 0 || t < P.delay - 1) return;
+	int64_t const slot = (t - P.delay + 1) % P.ring;
+	int64_t const nsp  = P.src_ring_cnt[slot];
+	int32_t const* sp  = P.src_ring_ids + slot * P.src_size;
+	int const lane     = threadIdx.x % kWave;
+	int64_t const gw   = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+	int64_t const nw   = int64_t(gridDim.x) * blockDim.x / kWave;
+	for (int64_t k = gw; k < nsp; k += nw) {
+		int64_t const src = sp[k];
+		int64_t const a = P.offsets[src], b = P.offsets[src + 1];
+		for (int64_t e = a + lane; e < b; e += kWave) {
+			uint32_t const pos = atomicAdd(P.ev_cursor + P.neighbors[e], 1u);
+			P.events[pos]      = uint32_t(e);
+		}
+	}
+}
+
 int64_t pop_tile(spice_ctx const* ctx, Pop const& P) { return P.tile ? P.tile : ctx->tile; }
 
 // Drops the cached window graphs (they hold the old descriptors and grids).
@@ -346,6 +425,7 @@ int refresh_device_descriptors(spice_ctx* ctx) {
 	SPICE_HIP(hipStreamSynchronize(ctx->stream));
 	if (ctx->mode == SPICE_DELIVER_PULL) {
 		for (auto& Q : ctx->projs) {
+			if (Q.dead) continue;
 			int64_t const t = pop_tile(ctx, ctx->pops[Q.dst]);
 			if (Q.tile != t)
 				if (int r = build_blocked(ctx, Q, t)) return r;
@@ -353,8 +433,11 @@ int refresh_device_descriptors(spice_ctx* ctx) {
 	}
 	std::vector<PopDev> pd;
 	std::vector<spice_pop_view> views;
+	ctx->max_pop_size = 0;
 	for (auto& P : ctx->pops) {
-		pd.push_back({P.inst, P.bitmap, P.bitmap_stride, P.size, P.ring_ids, P.ring_cnt});
+		pd.push_back({P.dead ? -1 : int64_t(P.inst), P.bitmap, P.bitmap_stride, P.size, P.ring_ids, P.ring_cnt,
+		              P.hist});
+		ctx->max_pop_size = std::max(ctx->max_pop_size, P.size);
 		spice_pop_view v;
 		std::memset(&v, 0, sizeof(v));
 		v.state         = P.state;
@@ -368,24 +451,58 @@ int refresh_device_descriptors(spice_ctx* ctx) {
 		v.nin           = int32_t(P.incoming.size());
 		for (size_t j = 0; j < P.incoming.size(); j++) {
 			Proj& Q = ctx->projs[P.incoming[j]];
-			v.in[j] = {Q.counts, Q.slots, P.incoming[j]};
+			v.in[j] = {Q.counts, Q.slots, P.incoming[j], Q.ev_offsets, Q.events, Q.syn_state, Q.nnz};
 		}
 		v.error        = ctx->error;
 		v.clear_counts = ctx->mode == SPICE_DELIVER_PUSH ? 1 : 0;
+		v.hist         = P.hist;
 		views.push_back(v);
 	}
 	std::vector<ProjDev> qd;
 	for (auto& Q : ctx->projs) {
 		Pop const& S = ctx->pops[Q.src];
-		qd.push_back({S.inst, Q.offsets, Q.neighbors, Q.counts, Q.slots, ctx->pops[Q.dst].size, Q.delay,
-		              S.ring_ids, S.ring_cnt, S.size, Q.rt, Q.btgt, Q.ntile});
+		qd.push_back({Q.dead ? -1 : int64_t(S.inst), Q.offsets, Q.neighbors, Q.counts, Q.slots, ctx->pops[Q.dst].size, Q.delay,
+		              S.ring_ids, S.ring_cnt, S.size, Q.rt, Q.btgt, Q.ntile, Q.ev_offsets});
 	}
+	// projection views for the synapse kernels; ids of stateful projections
+	std::vector<spice_proj_view> pvs;
+	std::vector<int64_t> stateful;
+	ctx->any_plastic = false;
+	for (size_t c = 0; c < ctx->projs.size(); c++) {
+		Proj& Q      = ctx->projs[c];
+		Pop const& S = ctx->pops[Q.src];
+		Pop const& D = ctx->pops[Q.dst];
+		spice_proj_view v;
+		std::memset(&v, 0, sizeof(v));
+		v.offsets      = Q.offsets;
+		v.neighbors    = Q.neighbors;
+		v.nnz          = Q.nnz;
+		v.src_size     = S.size;
+		v.dst_size     = D.size;
+		v.delay        = Q.delay;
+		v.ring         = ctx->ring;
+		v.src_ring_ids = S.ring_ids;
+		v.src_ring_cnt = S.ring_cnt;
+		v.syn_state    = Q.syn_state;
+		v.syn_words    = Q.syn_words;
+		v.inst         = Q.dead ? -1 : S.inst;
+		v.ages         = Q.ages;
+		v.dst_hist     = D.hist;
+		v.ev_cursor    = Q.ev_cursor;
+		v.events       = Q.events;
+		pvs.push_back(v);
+		if (Q.dead) continue;
+		if (Q.syn_state) stateful.push_back(int64_t(c));
+		ctx->any_plastic = ctx->any_plastic || Q.plastic;
+	}
+	ctx->nstateful = int64_t(stateful.size());
 	// pull tiles: every population with incoming projections, split into tiles
 	std::vector<PullTarget> pt;
 	std::vector<PullTile> tiles;
 	int max_in        = 0;
 	ctx->pull_max_tile = 64;
 	for (auto& P : ctx->pops) {
+		if (P.dead) continue;
 		max_in = std::max(max_in, int(P.incoming.size()));
 		if (!P.incoming.empty()) ctx->pull_max_tile = std::max(ctx->pull_max_tile, pop_tile(ctx, P));
 	}
@@ -398,7 +515,7 @@ int refresh_device_descriptors(spice_ctx* ctx) {
 		                              int(pull_lds_bytes(ctx->pull_max_tile, ctx->pull_pass))));
 	}
 	for (auto& P : ctx->pops) {
-		if (P.incoming.empty() || P.size == 0) continue;
+		if (P.dead || P.incoming.empty() || P.size == 0) continue;
 		PullTarget t{};
 		t.inst     = P.inst;
 		t.tile     = pop_tile(ctx, P);
@@ -414,6 +531,10 @@ int refresh_device_descriptors(spice_ctx* ctx) {
 	(void)hipFree(ctx->pops_dev);
 	(void)hipFree(ctx->projs_dev);
 	(void)hipFree(ctx->views_dev);
+	(void)hipFree(ctx->proj_views_dev);
+	(void)hipFree(ctx->stateful_dev);
+	ctx->proj_views_dev = nullptr;
+	ctx->stateful_dev   = nullptr;
 	ctx->pull_tiles   = nullptr;
 	ctx->pull_targets = nullptr;
 	ctx->pops_dev     = nullptr;
@@ -425,6 +546,14 @@ int refresh_device_descriptors(spice_ctx* ctx) {
 	if (int r = dev_alloc(&ctx->pops_dev, pd.size(), false)) return r;
 	if (int r = dev_alloc(&ctx->projs_dev, qd.size(), false)) return r;
 	if (int r = dev_alloc(&ctx->views_dev, views.size(), false)) return r;
+	if (int r = dev_alloc(&ctx->proj_views_dev, pvs.size(), false)) return r;
+	if (int r = dev_alloc(&ctx->stateful_dev, stateful.size(), false)) return r;
+	if (!pvs.empty())
+		SPICE_HIP(hipMemcpy(ctx->proj_views_dev, pvs.data(), sizeof(spice_proj_view) * pvs.size(),
+		                    hipMemcpyHostToDevice));
+	if (!stateful.empty())
+		SPICE_HIP(hipMemcpy(ctx->stateful_dev, stateful.data(), sizeof(int64_t) * stateful.size(),
+		                    hipMemcpyHostToDevice));
 	if (!tiles.empty())
 		SPICE_HIP(hipMemcpy(ctx->pull_tiles, tiles.data(), sizeof(PullTile) * tiles.size(), hipMemcpyHostToDevice));
 	if (!pt.empty())
@@ -504,7 +633,7 @@ int run_blocks(spice_ctx* ctx, int32_t nsteps, int32_t L, bool timed) {
 			SPICE_HIP(hipEventRecord(tb.ev[0], ctx->stream));
 		}
 		if (ctx->update_fn) {
-			spice_block const blk{ctx->win_dev, ctx->views_dev, rel, n};
+			spice_block const blk{ctx->win_dev, ctx->views_dev, ctx->proj_views_dev, rel, n, SPICE_PHASE_NEURONS};
 			if (ctx->update_fn(ctx->update_user, &blk, ctx->stream) != 0)
 				return set_error(SPICE_EINVAL, "neuron-update launch failed");
 		}
@@ -514,10 +643,27 @@ int run_blocks(spice_ctx* ctx, int32_t nsteps, int32_t L, bool timed) {
 			                   ctx->stream, ctx->pops_dev, ctx->win_dev, rel, ctx->ring);
 			SPICE_HIP(hipGetLastError());
 		}
+		if (ctx->any_plastic && ctx->max_pop_size > 0) {
+			hipLaunchKernelGGL(history_kernel, dim3(unsigned((ctx->max_pop_size + 255) / 256), unsigned(ctx->pops.size())),
+			                   dim3(256), 0, ctx->stream, ctx->pops_dev, ctx->win_dev, rel, n);
+			SPICE_HIP(hipGetLastError());
+		}
 		if (timed) SPICE_HIP(hipEventRecord(tb.ev[2], ctx->stream));
 		if (int r = launch_delivery(ctx, rel, n)) return r;
 		ctx->last_rel = rel;
 		ctx->last_n   = n;
+		if (ctx->nstateful > 0) {  // one-step blocks: event lists of the next step
+			hipLaunchKernelGGL(event_offsets_kernel, dim3(unsigned(ctx->nstateful)), dim3(1024), 0, ctx->stream,
+			                   ctx->proj_views_dev, ctx->stateful_dev, ctx->projs_dev, ctx->win_dev, rel);
+			hipLaunchKernelGGL(event_scatter_kernel, dim3(64, unsigned(ctx->nstateful)), dim3(256), 0,
+			                   ctx->stream, ctx->proj_views_dev, ctx->stateful_dev, ctx->win_dev, rel);
+			SPICE_HIP(hipGetLastError());
+		}
+		if (ctx->any_plastic && ctx->update_fn) {
+			spice_block const blk{ctx->win_dev, ctx->views_dev, ctx->proj_views_dev, rel, n, SPICE_PHASE_SYNAPSES};
+			if (ctx->update_fn(ctx->update_user, &blk, ctx->stream) != 0)
+				return set_error(SPICE_EINVAL, "synapse-update launch failed");
+		}
 		if (timed) {
 			SPICE_HIP(hipEventRecord(tb.ev[3], ctx->stream));
 			ctx->timed.push_back(tb);
@@ -569,6 +715,7 @@ void spice_ctx_destroy(spice_ctx* ctx) {
 		(void)hipFree(P.ring_ids);
 		(void)hipFree(P.ring_cnt);
 		(void)hipFree(P.chunk_jump);
+		(void)hipFree(P.hist);
 	}
 	for (auto& Q : ctx->projs) {
 		(void)hipFree(Q.offsets);
@@ -576,6 +723,11 @@ void spice_ctx_destroy(spice_ctx* ctx) {
 		(void)hipFree(Q.counts);
 		(void)hipFree(Q.rt);
 		(void)hipFree(Q.btgt);
+		(void)hipFree(Q.syn_state);
+		(void)hipFree(Q.ev_offsets);
+		(void)hipFree(Q.ev_cursor);
+		(void)hipFree(Q.events);
+		(void)hipFree(Q.ages);
 	}
 	(void)hipFree(ctx->pull_tiles);
 	(void)hipFree(ctx->pull_targets);
@@ -589,6 +741,8 @@ void spice_ctx_destroy(spice_ctx* ctx) {
 	(void)hipFree(ctx->win_steps_dev);
 	(void)hipFree(ctx->win_dev);
 	(void)hipFree(ctx->views_dev);
+	(void)hipFree(ctx->proj_views_dev);
+	(void)hipFree(ctx->stateful_dev);
 	(void)hipFree(ctx->pops_dev);
 	(void)hipFree(ctx->projs_dev);
 	(void)hipFree(ctx->error);
@@ -674,9 +828,49 @@ int spice_instance_create(spice_ctx* ctx, int64_t max_delay, int64_t* out_inst) 
 		return set_error(SPICE_EINVAL, "an instance's max_delay may not exceed its context's");
 	Instance I;
 	I.max_delay = max_delay;
-	ctx->insts.push_back(I);
 	ctx->dev_dirty = true;
-	*out_inst      = int64_t(ctx->insts.size()) - 1;
+	for (size_t i = 0; i < ctx->insts.size(); i++)  // reuse a destroyed instance's slot
+		if (!ctx->insts[i].live) {
+			ctx->insts[i] = I;
+			*out_inst     = int64_t(i);
+			return SPICE_OK;
+		}
+	ctx->insts.push_back(I);
+	*out_inst = int64_t(ctx->insts.size()) - 1;
+	return SPICE_OK;
+}
+
+int spice_instance_destroy(spice_ctx* ctx, int64_t inst) {
+	if (inst < 0 || inst >= int64_t(ctx->insts.size()) || !ctx->insts[inst].live)
+		return set_error(SPICE_EINVAL, "bad instance");
+	SPICE_HIP(hipSetDevice(ctx->device));
+	SPICE_HIP(hipStreamSynchronize(ctx->stream));
+	drop_graphs(ctx);
+	for (auto& Q : ctx->projs) {
+		if (Q.dead || ctx->pops[Q.src].inst != inst) continue;
+		for (void* p : {static_cast<void*>(Q.offsets), static_cast<void*>(Q.neighbors), static_cast<void*>(Q.counts),
+		                static_cast<void*>(Q.rt), static_cast<void*>(Q.btgt), static_cast<void*>(Q.syn_state),
+		                static_cast<void*>(Q.ev_offsets), static_cast<void*>(Q.ev_cursor),
+		                static_cast<void*>(Q.events), static_cast<void*>(Q.ages)})
+			(void)hipFree(p);
+		Proj dead;
+		dead.dead = true;
+		dead.src  = Q.src;
+		dead.dst  = Q.dst;
+		Q         = dead;
+	}
+	for (auto& P : ctx->pops) {
+		if (P.dead || P.inst != inst) continue;
+		for (void* p : {static_cast<void*>(P.state), static_cast<void*>(P.bitmap), static_cast<void*>(P.ring_ids),
+		                static_cast<void*>(P.ring_cnt), static_cast<void*>(P.chunk_jump), static_cast<void*>(P.hist)})
+			(void)hipFree(p);
+		Pop dead;
+		dead.dead = true;
+		dead.inst = P.inst;
+		P         = dead;
+	}
+	ctx->insts[inst].live = false;
+	ctx->dev_dirty        = true;
 	return SPICE_OK;
 }
 
@@ -689,7 +883,8 @@ int64_t spice_instance_count(spice_ctx* ctx) { return int64_t(ctx->insts.size())
 
 int spice_pop_create_in(spice_ctx* ctx, int64_t inst, int64_t size, int32_t state_words, int32_t draws,
                         int64_t* out_pop) {
-	if (inst < 0 || inst >= int64_t(ctx->insts.size())) return set_error(SPICE_EINVAL, "bad instance");
+	if (inst < 0 || inst >= int64_t(ctx->insts.size()) || !ctx->insts[inst].live)
+		return set_error(SPICE_EINVAL, "bad instance");
 	if (size < 0) return precondition("neuron_population.h", 35, "size >= 0");
 	if (size >= INT32_MAX) return precondition("topology.cpp", 35, "size < INT32_MAX");
 	if (state_words < 0 || draws < 0) return set_error(SPICE_EINVAL, "negative state words or draws");
@@ -785,7 +980,8 @@ int spice_pop_download(spice_ctx* ctx, int64_t pop, void* aos, int32_t struct_by
 }
 
 static int proj_common(spice_ctx* ctx, int64_t src, int64_t dst, int64_t delay, Proj& Q) {
-	if (src < 0 || dst < 0 || src >= int64_t(ctx->pops.size()) || dst >= int64_t(ctx->pops.size()))
+	if (src < 0 || dst < 0 || src >= int64_t(ctx->pops.size()) || dst >= int64_t(ctx->pops.size()) ||
+	    ctx->pops[src].dead || ctx->pops[dst].dead)
 		return set_error(SPICE_EINVAL, "bad population id");
 	if (!(delay >= 1)) return precondition("snn.h", 35, "d >= 1 && \"The delay must be at least 1dt.\"");
 	if (!(delay <= ctx->insts[ctx->pops[src].inst].max_delay))
@@ -860,7 +1056,8 @@ int spice_proj_synapses(spice_ctx* ctx, int64_t proj, int64_t* out) {
 }
 
 int spice_proj_download(spice_ctx* ctx, int64_t proj, int64_t* offsets, int32_t* neighbors) {
-	if (proj < 0 || proj >= int64_t(ctx->projs.size())) return set_error(SPICE_EINVAL, "bad projection");
+	if (proj < 0 || proj >= int64_t(ctx->projs.size()) || ctx->projs[proj].dead)
+		return set_error(SPICE_EINVAL, "bad projection");
 	SPICE_HIP(hipSetDevice(ctx->device));
 	Proj& Q         = ctx->projs[proj];
 	int64_t const S = ctx->pops[Q.src].size;
@@ -868,6 +1065,104 @@ int spice_proj_download(spice_ctx* ctx, int64_t proj, int64_t* offsets, int32_t*
 	if (offsets) SPICE_HIP(hipMemcpy(offsets, Q.offsets, sizeof(int64_t) * (S + 1), hipMemcpyDeviceToHost));
 	if (neighbors && Q.nnz)
 		SPICE_HIP(hipMemcpy(neighbors, Q.neighbors, sizeof(int32_t) * Q.nnz, hipMemcpyDeviceToHost));
+	return SPICE_OK;
+}
+
+__global__ void __launch_bounds__(256)
+fill_synapses_kernel(uint32_t* state, int64_t nnz, int32_t words, uint32_t const* one) {
+	int64_t const stride = int64_t(gridDim.x) * blockDim.x;
+	for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < nnz; e += stride)
+		for (int w = 0; w < words; w++) state[w * nnz + e] = one[w];
+}
+
+int spice_proj_set_synapses(spice_ctx* ctx, int64_t proj, void const* aos, int64_t count, int32_t struct_bytes) {
+	if (proj < 0 || proj >= int64_t(ctx->projs.size())) return set_error(SPICE_EINVAL, "bad projection");
+	if (struct_bytes <= 0 || struct_bytes > 256) return set_error(SPICE_EINVAL, "synapse struct of 1 to 256 bytes");
+	Proj& Q = ctx->projs[proj];
+	if (count != 1 && count != Q.nnz) return set_error(SPICE_EINVAL, "give one synapse struct or one per synapse");
+	if (Q.nnz >= (int64_t(1) << 32)) return set_error(SPICE_EINVAL, "stateful projections hold fewer than 2^32 synapses");
+	SPICE_HIP(hipSetDevice(ctx->device));
+	SPICE_HIP(hipStreamSynchronize(ctx->stream));
+	int32_t const words = (struct_bytes + 3) / 4;
+	int64_t const D     = ctx->pops[Q.dst].size;
+	if (!Q.syn_state) {
+		if (int r = dev_alloc(&Q.syn_state, size_t(words) * size_t(std::max<int64_t>(Q.nnz, 1)), false)) return r;
+		if (int r = dev_alloc(&Q.ev_offsets, size_t(std::max<int64_t>(D, 1)))) return r;
+		if (int r = dev_alloc(&Q.ev_cursor, size_t(std::max<int64_t>(D, 1)))) return r;
+		if (int r = dev_alloc(&Q.events, size_t(std::max<int64_t>(Q.nnz, 1)), false)) return r;
+	} else if (Q.syn_words != words) {
+		return set_error(SPICE_EINVAL, "synapse struct size changed");
+	}
+	Q.syn_words = words;
+	auto const* src = static_cast<unsigned char const*>(aos);
+	if (count == 1) {
+		uint32_t one[64] = {};
+		std::memcpy(one, src, size_t(struct_bytes));
+		uint32_t* d_one = nullptr;
+		if (int r = dev_alloc(&d_one, 64, false)) return r;
+		SPICE_HIP(hipMemcpy(d_one, one, sizeof(one), hipMemcpyHostToDevice));
+		if (Q.nnz > 0)
+			hipLaunchKernelGGL(fill_synapses_kernel, dim3(unsigned(std::min<int64_t>((Q.nnz + 255) / 256, 65536))),
+			                   dim3(256), 0, ctx->stream, Q.syn_state, Q.nnz, words, d_one);
+		SPICE_HIP(hipGetLastError());
+		SPICE_HIP(hipStreamSynchronize(ctx->stream));
+		(void)hipFree(d_one);
+	} else if (Q.nnz > 0) {
+		std::vector<uint32_t> soa(size_t(words) * size_t(Q.nnz), 0);
+		for (int64_t e = 0; e < Q.nnz; e++) {
+			uint32_t w[64] = {};
+			std::memcpy(w, src + size_t(e) * size_t(struct_bytes), size_t(struct_bytes));
+			for (int k = 0; k < words; k++) soa[size_t(k) * size_t(Q.nnz) + size_t(e)] = w[k];
+		}
+		SPICE_HIP(hipMemcpy(Q.syn_state, soa.data(), sizeof(uint32_t) * soa.size(), hipMemcpyHostToDevice));
+	}
+	ctx->dev_dirty = true;
+	return SPICE_OK;
+}
+
+int spice_proj_get_synapses(spice_ctx* ctx, int64_t proj, void* aos, int32_t struct_bytes) {
+	if (proj < 0 || proj >= int64_t(ctx->projs.size())) return set_error(SPICE_EINVAL, "bad projection");
+	Proj& Q = ctx->projs[proj];
+	if (!Q.syn_state) return set_error(SPICE_EINVAL, "projection without per-synapse state");
+	if ((struct_bytes + 3) / 4 != Q.syn_words) return set_error(SPICE_EINVAL, "synapse struct size mismatch");
+	SPICE_HIP(hipSetDevice(ctx->device));
+	SPICE_HIP(hipStreamSynchronize(ctx->stream));
+	std::vector<uint32_t> soa(size_t(Q.syn_words) * size_t(std::max<int64_t>(Q.nnz, 1)));
+	if (Q.nnz)
+		SPICE_HIP(hipMemcpy(soa.data(), Q.syn_state, sizeof(uint32_t) * size_t(Q.syn_words) * size_t(Q.nnz),
+		                    hipMemcpyDeviceToHost));
+	auto* dst = static_cast<unsigned char*>(aos);
+	for (int64_t e = 0; e < Q.nnz; e++) {
+		uint32_t w[64];
+		for (int k = 0; k < Q.syn_words; k++) w[k] = soa[size_t(k) * size_t(Q.nnz) + size_t(e)];
+		std::memcpy(dst + size_t(e) * size_t(struct_bytes), w, size_t(struct_bytes));
+	}
+	return SPICE_OK;
+}
+
+int spice_proj_set_plastic(spice_ctx* ctx, int64_t proj) {
+	if (proj < 0 || proj >= int64_t(ctx->projs.size())) return set_error(SPICE_EINVAL, "bad projection");
+	Proj& Q = ctx->projs[proj];
+	if (!Q.syn_state) return set_error(SPICE_EINVAL, "plastic synapses need per-synapse state");
+	if (Q.plastic) return SPICE_OK;
+	SPICE_HIP(hipSetDevice(ctx->device));
+	if (int r = dev_alloc(&Q.ages, size_t(std::max<int64_t>(ctx->pops[Q.src].size, 1)))) return r;
+	// the reference keeps the source's history (snn.h:46-47) and reads the
+	// target's (synapse_population.h:97-100); both are kept here
+	for (int64_t pop : {Q.src, Q.dst}) {
+		Pop& P = ctx->pops[pop];
+		if (!P.hist)
+			if (int r = dev_alloc(&P.hist, size_t(std::max<int64_t>(P.size, 1)))) return r;
+	}
+	Q.plastic      = true;
+	ctx->dev_dirty = true;
+	return SPICE_OK;
+}
+
+int spice_ctx_proj_views(spice_ctx* ctx, spice_proj_view const** out) {
+	SPICE_HIP(hipSetDevice(ctx->device));
+	if (int r = refresh_device_descriptors(ctx)) return r;
+	*out = ctx->proj_views_dev;
 	return SPICE_OK;
 }
 
@@ -989,7 +1284,7 @@ int spice_run(spice_ctx* ctx, int32_t nsteps, spice_step const* steps) {
 		                         sizeof(spice_step) * size_t(nsteps), hipMemcpyHostToDevice, ctx->stream));
 	SPICE_HIP(hipEventRecord(ctx->win_copied[b], ctx->stream));
 
-	int32_t L = SPICE_MAX_BLOCK;
+	int32_t L = ctx->nstateful > 0 ? 1 : SPICE_MAX_BLOCK;  // event lists are built step by step
 	for (auto& Q : ctx->projs) L = int32_t(std::min<int64_t>(L, Q.delay));
 	if (ctx->graphs && !ctx->timing) {
 		auto it = ctx->graph_cache.find(nsteps);
@@ -1019,14 +1314,15 @@ int spice_run(spice_ctx* ctx, int32_t nsteps, spice_step const* steps) {
 }
 
 int64_t spice_history_len(spice_ctx* ctx, int64_t pop) {
-	if (pop < 0 || pop >= int64_t(ctx->pops.size())) return 0;
+	if (pop < 0 || pop >= int64_t(ctx->pops.size()) || ctx->pops[pop].dead) return 0;
 	Instance const& I = ctx->insts[ctx->pops[pop].inst];
 	return std::min<int64_t>(I.steps, I.max_delay);
 }
 
 int spice_spikes(spice_ctx* ctx, int64_t pop, int64_t age, int32_t* out, int64_t cap,
                  int64_t* out_count) {
-	if (pop < 0 || pop >= int64_t(ctx->pops.size())) return set_error(SPICE_EINVAL, "bad population");
+	if (pop < 0 || pop >= int64_t(ctx->pops.size()) || ctx->pops[pop].dead)
+		return set_error(SPICE_EINVAL, "bad population");
 	if (!(0 <= age && age < spice_history_len(ctx, pop)))
 		return precondition("neuron_population.h", 148, "0 <= age && age < _spike_counts.size()");
 	SPICE_HIP(hipSetDevice(ctx->device));

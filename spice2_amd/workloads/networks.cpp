@@ -5,6 +5,7 @@
 // samples/vogels.cpp:10-59) with their floating-point expressions unchanged;
 // the Poisson rate is a member so that swept rates reuse the same struct (at
 // 20 Hz it computes exactly what the sample's constant does).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <exception>
@@ -12,6 +13,7 @@
 #include <string>
 #include <vector>
 
+#include "spice/serve.h"
 #include "spice/snn.h"
 
 using namespace spice;
@@ -48,6 +50,60 @@ struct lif {
 struct fixed_weight {
 	float weight;
 	void deliver(lif::neuron& to) const { to.V += weight; }
+};
+
+// samples/brunel+.cpp:59-99
+struct plastic {
+	struct synapse {
+		float W     = 1e-4;
+		float Zpre  = 0;
+		float Zpost = 0;
+	};
+	void deliver(synapse const& syn, lif::neuron& to) const { to.V += syn.W; }
+	void update(synapse& syn, float const dt, bool const pre, bool const post) const {
+		float const TstdpInv = 1.0f / 0.02f;
+		float const dtInv    = 1.0f / dt;
+		syn.W = std::clamp(syn.W - pre * 0.0202f * syn.W * std::exp(-syn.Zpost * dtInv) +
+		                       post * 0.01f * (1.0f - syn.W) * std::exp(-syn.Zpre * dtInv),
+		                   0.0f, 0.0003f);
+		syn.Zpre += pre;
+		syn.Zpost += post;
+		syn.Zpre -= syn.Zpre * dt * TstdpInv;
+		syn.Zpost -= syn.Zpost * dt * TstdpInv;
+	}
+	void skip(synapse& syn, float const dt, Int const n) const {
+		float const TstdpInv = 1.0f / 0.02f;
+		syn.Zpre *= std::pow(1 - dt * TstdpInv, n);
+		syn.Zpost *= std::pow(1 - dt * TstdpInv, n);
+	}
+};
+
+// Pair STDP on exponential traces: tau 20 ms, A+ 0.01 and A- 0.0105 relative
+// to the initial weight w0, W clipped to [0, 2 w0].
+struct stdp_pair {
+	float w0;
+	struct synapse {
+		float W     = 0;
+		float Zpre  = 0;
+		float Zpost = 0;
+	};
+	void init(synapse& syn, Int, Int, auto&) const { syn.W = w0; }
+	void deliver(synapse const& syn, lif::neuron& to) const { to.V += syn.W; }
+	void update(synapse& syn, float const dt, bool const pre, bool const post) const {
+		float const tauInv = 1.0f / 0.02f;
+		float const wmax   = 2.0f * w0;
+		if (pre) syn.W = std::clamp(syn.W - 0.0105f * w0 * syn.Zpost, 0.0f, wmax);
+		if (post) syn.W = std::clamp(syn.W + 0.01f * w0 * syn.Zpre, 0.0f, wmax);
+		syn.Zpre += pre;
+		syn.Zpost += post;
+		syn.Zpre -= syn.Zpre * dt * tauInv;
+		syn.Zpost -= syn.Zpost * dt * tauInv;
+	}
+	void skip(synapse& syn, float const dt, Int const n) const {
+		float const tauInv = 1.0f / 0.02f;
+		syn.Zpre *= std::pow(1 - dt * tauInv, n);
+		syn.Zpost *= std::pow(1 - dt * tauInv, n);
+	}
 };
 
 struct coba {
@@ -184,6 +240,37 @@ void* wl_coba_create(int64_t nE, int64_t nI, double p, float wex, float win, flo
 	}
 }
 
+// Brunel+ (samples/brunel+.cpp:101-117): the E->E projection is plastic, with
+// rule 1 (the sample's `plastic`) or rule 2 (stdp_pair with w0).
+void* wl_brunel_plastic_create(int64_t nP, int64_t nE, int64_t nI, double p, float rate, float wex, float win,
+                               float dt, float delay, uint32_t const* seeds, int nseeds, int rule, float w0,
+                               int mode) {
+	try {
+		auto w  = std::make_unique<network>();
+		w->net  = std::make_unique<snn>(dt, delay, make_seed(seeds, nseeds));
+		auto& s = *w->net;
+		s.set_delivery(mode);
+		auto P = s.add_population<wl::poisson>(nP, {rate});
+		auto E = s.add_population<wl::lif>(nE);
+		auto I = s.add_population<wl::lif>(nI);
+		s.connect<wl::fixed_weight>(P, E, fixed_probability(p), delay, {wex});
+		s.connect<wl::fixed_weight>(P, I, fixed_probability(p), delay, {wex});
+		if (rule == 1)
+			s.connect<wl::plastic>(E, E, fixed_probability(p), delay);
+		else
+			s.connect<wl::stdp_pair>(E, E, fixed_probability(p), delay, {w0});
+		s.connect<wl::fixed_weight>(E, I, fixed_probability(p), delay, {wex});
+		s.connect<wl::fixed_weight>(I, E, fixed_probability(p), delay, {win});
+		s.connect<wl::fixed_weight>(I, I, fixed_probability(p), delay, {win});
+		w->pops = {P, E, I};
+		w->kind = 0;
+		return w.release();
+	} catch (std::exception const& e) {
+		fail(e);
+		return nullptr;
+	}
+}
+
 void wl_destroy(void* h) { delete static_cast<network*>(h); }
 
 int wl_step(void* h, int64_t n) {
@@ -311,6 +398,118 @@ int wl_batch_spikes(void* h, int inst, int pop, int64_t age, int32_t* out, int64
 }
 
 spice_ctx* wl_batch_context(void* h) { return static_cast<brunel_batch*>(h)->b->context(); }
+
+// ---- served job stream -----------------------------------------------------
+// Job kinds: 0 Brunel (P/E/I = N/2, 4N/10, N/10, delay 15 steps, weights
+// +0.2/(N p), -1/(N p)), 1 conductance E/I (E/I = 4N/5, N/5, delay 1 step,
+// V0 ~ U[-60,-50] mV, weights 0.27 and 4.5 (x 80/(N p)) of the leak conductance,
+// as in samples/vogels.cpp), 2 Brunel whose E->E projection is pair-STDP plastic.
+// The result of a job is an FNV-1a hash of its stateful populations' neurons
+// after its last step (AoS, population order) plus the spike counts of that step.
+namespace {
+uint64_t fnv1a(uint64_t h, void const* p, size_t n) {
+	auto const* b = static_cast<unsigned char const*>(p);
+	for (size_t i = 0; i < n; i++) {
+		h ^= b[i];
+		h *= 0x100000001b3ull;
+	}
+	return h;
+}
+
+std::vector<spice::detail::NeuronPopulation*> build_job(snn& s, int kind, int64_t N, double p, float delay) {
+	if (kind == 0) {
+		float const wex = float(0.2 / (double(N) * p)), win = float(-1.0 / (double(N) * p));
+		return build_brunel(s, N / 2, N * 4 / 10, N / 10, p, 20.0f, wex, win, delay);
+	}
+	if (kind == 1) {
+		float const scale = float(80.0 / (double(N) * p));
+		auto E            = s.add_population<wl::coba>(N * 4 / 5, {1});
+		auto I            = s.add_population<wl::coba>(N / 5, {1});
+		s.connect<wl::excitatory>(E, E, fixed_probability(p), delay, {0.27f * scale});
+		s.connect<wl::excitatory>(E, I, fixed_probability(p), delay, {0.27f * scale});
+		s.connect<wl::inhibitory>(I, E, fixed_probability(p), delay, {4.5f * scale});
+		s.connect<wl::inhibitory>(I, I, fixed_probability(p), delay, {4.5f * scale});
+		return {E, I};
+	}
+	float const wex = float(0.2 / (double(N) * p)), win = float(-1.0 / (double(N) * p));
+	auto P = s.add_population<wl::poisson>(N / 2, {20.0f});
+	auto E = s.add_population<wl::lif>(N * 4 / 10);
+	auto I = s.add_population<wl::lif>(N / 10);
+	s.connect<wl::fixed_weight>(P, E, fixed_probability(p), delay, {wex});
+	s.connect<wl::fixed_weight>(P, I, fixed_probability(p), delay, {wex});
+	s.connect<wl::stdp_pair>(E, E, fixed_probability(p), delay, {wex});
+	s.connect<wl::fixed_weight>(E, I, fixed_probability(p), delay, {wex});
+	s.connect<wl::fixed_weight>(I, E, fixed_probability(p), delay, {win});
+	s.connect<wl::fixed_weight>(I, I, fixed_probability(p), delay, {win});
+	return {P, E, I};
+}
+
+uint64_t job_result(int kind, std::vector<spice::detail::NeuronPopulation*> const& pops) {
+	uint64_t h = 0xcbf29ce484222325ull;
+	for (auto* p : pops) {
+		void* n = p->neurons();
+		if (n) h = fnv1a(h, n, size_t(p->size()) * (kind == 1 ? sizeof(wl::coba::neuron) : sizeof(wl::lif::neuron)));
+	}
+	for (auto* p : pops) {
+		int64_t const c = int64_t(p->spikes(0).size());
+		h               = fnv1a(h, &c, sizeof(c));
+	}
+	return h;
+}
+
+float job_delay(int kind) { return kind == 1 ? 1e-4f : 15e-4f; }
+}  // namespace
+
+// Runs n jobs through spice::server (device-memory budget `budget` bytes,
+// windows of `window` steps).  Per job: kind, N, p, steps, arrival (s), seed
+// words seeds[2i..2i+1] and a footprint estimate in bytes.  Out per job:
+// admitted/finished times (s) and the result hash; *events = synaptic events.
+int wl_serve_run(int n, int const* kind, int64_t const* N, double const* p, int64_t const* steps,
+                 double const* arrival, uint32_t const* seeds, int64_t const* footprint, int64_t budget,
+                 int64_t window, double* admitted, double* finished, uint64_t* result, int64_t* events) {
+	try {
+		spice::server srv(budget, window, 15);
+		for (int i = 0; i < n; i++) {
+			spice::request r;
+			r.max_delay   = job_delay(kind[i]);
+			r.seed        = make_seed(seeds + 2 * i, 2);
+			r.steps       = steps[i];
+			r.footprint   = footprint[i];
+			r.single_step = kind[i] != 0;
+			r.arrival     = arrival[i];
+			auto pops     = std::make_shared<std::vector<spice::detail::NeuronPopulation*>>();
+			int const k   = kind[i];
+			int64_t const Ni = N[i];
+			double const pi  = p[i];
+			r.build  = [=](snn& s) { *pops = build_job(s, k, Ni, pi, job_delay(k)); };
+			r.finish = [=](snn&) { result[i] = job_result(k, *pops); };
+			srv.submit(std::move(r));
+		}
+		auto recs = srv.run();
+		for (int i = 0; i < n; i++) {
+			admitted[i] = recs[size_t(i)].admitted;
+			finished[i] = recs[size_t(i)].finished;
+		}
+		*events = srv.events();
+		return 0;
+	} catch (std::exception const& e) {
+		return fail(e);
+	}
+}
+
+// One job of wl_serve_run on a network of its own (the reference's way):
+// the result hash, for comparison with the served run.
+int wl_job_alone(int kind, int64_t N, double p, int64_t steps, uint32_t const* seed2, uint64_t* result) {
+	try {
+		snn s(1e-4f, job_delay(kind), make_seed(seed2, 2));
+		auto pops = build_job(s, kind, N, p, job_delay(kind));
+		for (int64_t t = 0; t < steps; t++) s.step();
+		*result = job_result(kind, pops);
+		return 0;
+	} catch (std::exception const& e) {
+		return fail(e);
+	}
+}
 
 int wl_batch_set_window(void* h, int64_t steps) {
 	try {
