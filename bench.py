@@ -220,6 +220,48 @@ def run_batch(n: int, steps: int, window: int, plain_sample: int):
                      "seeds {1337, i}, numpy PCG64(2024)"}
 
 
+def serve_requests(n: int, rate: float, seed: int = 2025):
+    """The served job stream: kinds uniform over Brunel (N log-uniform in
+    [10k, 200k], p = min(0.1, 2000/N)), conductance E/I (N in [10k, 100k],
+    p = min(0.1, 1000/N), delay 1) and Brunel with pair-STDP E->E (N in
+    [10k, 200k], p = min(0.1, 1000/N)); 500-5000 steps; Poisson arrivals at
+    `rate` per second; seeds {2025, i} (numpy PCG64(seed))."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    kind = rng.integers(0, 3, n)
+    hi = np.where(kind == 1, 1e5, 2e5)
+    N = np.round(np.exp(rng.uniform(np.log(1e4), np.log(hi)))).astype(np.int64)
+    p = np.minimum(0.1, np.where(kind == 0, 2000.0, 1000.0) / N)
+    steps = rng.integers(500, 5001, n)
+    arrival = np.cumsum(rng.exponential(1.0 / rate, n))
+    return kind, N, p, steps, arrival, [(2025, i) for i in range(n)]
+
+
+def run_serve(n: int, rate: float, budget_gb: float, window: int, alone_sample: int):
+    """The job stream through spice::server; p50/p99 latency (arrival to
+    finish), aggregate synaptic events/s over the run, and a sample of jobs
+    re-run alone whose results must be identical."""
+    import spice2_amd as S
+    kind, N, p, steps, arr, seeds = serve_requests(n, rate)
+    fp = [S.job_footprint(int(k), int(x), float(q)) for k, x, q in zip(kind, N, p)]
+    t0 = time.perf_counter()
+    adm, fin, res, ev = S.serve(kind, N, p, steps, arr, seeds, fp, int(budget_gb * 2**30), window)
+    wall = time.perf_counter() - t0
+    lat = fin - arr
+    sample = np.linspace(0, n - 1, alone_sample).round().astype(int)
+    same = all(res[i] == S.job_alone(int(kind[i]), int(N[i]), float(p[i]), int(steps[i]), seeds[i])
+               for i in sample)
+    return {"requests": n, "arrival_rate_per_s": rate, "budget_gb": budget_gb, "window": window,
+            "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
+            "queue_wait_p50_s": float(np.percentile(adm - arr, 50)),
+            "events": int(ev), "makespan_s": float(fin.max()), "wall_s": wall,
+            "value": ev / float(fin.max()), "unit": "synaptic events/s",
+            "kinds": [int((kind == k).sum()) for k in range(3)],
+            "sample_alone": len(sample), "sample_results_identical": bool(same),
+            "mix": "Brunel N 10k-200k p=min(0.1,2000/N) | conductance E/I N 10k-100k p=min(0.1,1000/N) "
+                   "delay 1 | Brunel + pair-STDP E->E N 10k-200k p=min(0.1,1000/N); steps U[500,5000]; "
+                   "Poisson arrivals; numpy PCG64(2025)"}
+
+
 def run_mode(mode: int, steps: int, warmup: int, tile: int | None = None):
     import spice2_amd as S
 
@@ -260,6 +302,12 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument("--batch-window", type=int, default=100)
     ap.add_argument("--plain-sample", type=int, default=16)
+    ap.add_argument("--no-serve", action="store_true", help="skip the served-stream leg")
+    ap.add_argument("--serve-requests", type=int, default=1000)
+    ap.add_argument("--serve-rate", type=float, default=40.0, help="arrivals per second")
+    ap.add_argument("--serve-budget-gb", type=float, default=160.0)
+    ap.add_argument("--serve-window", type=int, default=100)
+    ap.add_argument("--serve-alone", type=int, default=6)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the known-size copy kernel (PMC counter calibration)")
     ap.add_argument("--traffic-summary", nargs=5, metavar=("FETCH_RUN", "WRITE_RUN", "CALIB_FETCH",
@@ -297,6 +345,10 @@ def main():
     batch = None
     if not args.no_batch:
         batch = run_batch(args.batch_instances, args.batch_steps, args.batch_window, args.plain_sample)
+    served = None
+    if not args.no_serve:
+        served = run_serve(args.serve_requests, args.serve_rate, args.serve_budget_gb, args.serve_window,
+                           args.serve_alone)
 
     line = {
         "metric": "synaptic events/s (Brunel N=500k, p=0.02, deterministic rasters)",
@@ -328,6 +380,7 @@ def main():
                      "avg_launch_ms": avg_ms},
         "cpu_baseline": cpu,
         "batched": batch,
+        "served": served,
     }
     print(json.dumps(line), flush=True)
 

@@ -307,6 +307,32 @@ class Network:
         self._check(self.L.wl_set_graphs(self.h, int(on)))
 
 
+def job_footprint(kind: int, N: int, p: float, tile: int = 8192) -> int:
+    """Device bytes a served job holds (admission estimate): per synapse 4 B
+    target + 2 B tile-local target (+ 12 B state and 4 B event slot for the
+    plastic E->E projection); per source and projection 8 B offset and
+    2 B per tile run start; per neuron its state, a 31-slot spike ring and
+    31-slot event counters per incoming projection."""
+    ring = 15 + 16
+    if kind == 1:
+        E, I = N * 4 // 5, N // 5
+        pops = [(E, 16, 2), (I, 16, 2)]
+        projs = [(E, E), (E, I), (I, E), (I, I)]
+    else:
+        P_, E, I = N // 2, N * 4 // 10, N // 10
+        pops = [(P_, 0, 0), (E, 8, 3), (I, 8, 3)]
+        projs = [(P_, E), (P_, I), (E, E), (E, I), (I, E), (I, I)]
+    total = 0
+    for a, b in projs:
+        nnz = a * b * p
+        total += nnz * 6 + a * (8 + 2 * (b // tile + 2))
+    if kind == 2:
+        total += (N * 4 // 10) ** 2 * p * 16
+    for n, sb, nin in pops:
+        total += n * (sb + 4 * ring + 4 * ring * nin + 8 * 16 // 64 + 8)
+    return int(total * 1.1) + (1 << 20)
+
+
 def serve(kind, N, p, steps, arrival, seeds, footprint, budget, window=100):
     """Runs jobs through spice::server (libspice_workloads.so, wl_serve_run).
     Returns (admitted, finished, result hashes, synaptic events)."""

@@ -34,6 +34,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -169,6 +170,7 @@ struct ChainJob {
 	uint64_t const* block_prefix;  // nblk + 1 entries
 	int64_t* row_start;            // S + 1 entries
 	int64_t r0, s0;                // resume point
+	int32_t active;                // walk this launch
 	int32_t status;                // 0 done, 1 host needed at row `status_row`, 2 error
 	int64_t status_row;
 	unsigned long long ambiguous;
@@ -193,6 +195,7 @@ __device__ uint64_t prefix_at(FpParams const& P, xoro_state const* block_state,
 __global__ void __launch_bounds__(64) fp_chain_kernel(ChainJob* jobs) {
 	__shared__ double lds_lo[1024], lds_hi[1024];
 	ChainJob& J      = jobs[blockIdx.x];
+	if (!J.active) return;
 	FpParams const P = J.P;
 	int const lane   = threadIdx.x;
 	int64_t s        = J.s0;
@@ -364,20 +367,27 @@ struct DevBuf {
 
 }  // namespace
 
-int generate_fixed_probability(spice_ctx* ctx, Proj& pr, int64_t S, int64_t D, double p,
-                               xoro_state seed) {
-	auto const t_begin = std::chrono::steady_clock::now();
-	hipStream_t const st = ctx->stream;
+// One projection being generated: parameters and device temporaries.
+struct GenJob {
+	int64_t proj = -1;
+	int64_t S = 0, D = 0;
+	FpParams P{};
+	xoro_state seed{};
+	int64_t L = 0;
+	DevBuf<xoro_state> groups, block_state;
+	DevBuf<uint64_t> prefix, tiles, total;
+	DevBuf<int64_t> row_start, unc;
+	DevBuf<int32_t> nunc;
+	DevBuf<unsigned long long> amb;
+	ChainJob job{};
+	int64_t host_rows = 0;
+};
+
+// The parameters of topology.cpp:73-112 for one projection, checked on the host.
+static int fp_params(int64_t S, int64_t D, double p, FpParams& P, int64_t& L) {
 	// max_degree exactly as topology.cpp:74-75 (double arithmetic, truncation)
 	int64_t const M = int64_t(double(D) * p + 3 * std::sqrt(double(D) * p * (1 - p)));
-	SPICE_HIP(hipMalloc(reinterpret_cast<void**>(&pr.offsets), sizeof(int64_t) * (S + 1)));
-	if (D == 0 || p == 0) {  // topology.cpp:84-85: all rows empty
-		SPICE_HIP(hipMemsetAsync(pr.offsets, 0, sizeof(int64_t) * (S + 1), st));
-		SPICE_HIP(hipMalloc(reinterpret_cast<void**>(&pr.neighbors), sizeof(int32_t)));
-		pr.nnz = 0;
-		return SPICE_OK;
-	}
-	FpParams P{};
+	P       = FpParams{};
 	P.S     = S;
 	P.D     = D;
 	P.M     = M;
@@ -401,142 +411,189 @@ int generate_fixed_probability(spice_ctx* ctx, Proj& pr, int64_t S, int64_t D, d
 	// SPICE_CSR_EXACT_ROWS=1 decides every row's length with the exact interval
 	// evaluation instead of the fixed-point probe (used to test that path).
 	if (char const* e = std::getenv("SPICE_CSR_EXACT_ROWS"); e && e[0] == '1') P.tol = UINT64_MAX / 2;
-	int64_t const L    = S * (M + 1);  // worst-case stream length
+	L = S * (M + 1);  // worst-case stream length
 	// every kernel reads positions below L + 1024 (a row evaluation may run one
 	// 1024-position round past the last row's end)
-	P.nblk             = (L + 1024 + 63) / 64 + 1;
+	P.nblk = (L + 1024 + 63) / 64 + 1;
+	return SPICE_OK;
+}
 
-	// phase A
-	int64_t const ngroups = (P.nblk + kGroup - 1) / kGroup;
-	std::vector<xoro_state> groups(ngroups);
-	{
-		gf2poly const gstep = xoro_jump_poly(uint64_t(64) * kGroup);
-		xoro_state g        = seed;
-		for (int64_t i = 0; i < ngroups; i++) {
-			groups[i] = g;
-			g         = xoro_jump(g, gstep);
-		}
+int queue_fixed_probability(spice_ctx* ctx, int64_t proj, int64_t S, int64_t D, double p, xoro_state seed) {
+	Proj& pr = ctx->projs[proj];
+	SPICE_HIP(hipMalloc(reinterpret_cast<void**>(&pr.offsets), sizeof(int64_t) * (S + 1)));
+	if (D == 0 || p == 0) {  // topology.cpp:84-85: all rows empty
+		SPICE_HIP(hipMemsetAsync(pr.offsets, 0, sizeof(int64_t) * (S + 1), ctx->stream));
+		SPICE_HIP(hipMalloc(reinterpret_cast<void**>(&pr.neighbors), sizeof(int32_t)));
+		pr.nnz = 0;
+		return SPICE_OK;
 	}
-	DevBuf<xoro_state> d_groups, d_block_state;
+	FpParams P;
+	int64_t L;
+	if (int r = fp_params(S, D, p, P, L)) return r;
+	ctx->pending_fp.push_back({proj, S, D, p, seed});
+	pr.nnz = -1;  // not generated yet
+	return SPICE_OK;
+}
+
+// Generates every queued fixed_probability projection.  Phases A and C run
+// projection after projection (each fills the device); the row-chain walks of
+// phase B, one workgroup each, run side by side in one launch.
+int flush_fixed_probability(spice_ctx* ctx) {
+	if (ctx->pending_fp.empty()) return SPICE_OK;
+	auto const t_begin = std::chrono::steady_clock::now();
+	hipStream_t const st = ctx->stream;
+	std::vector<PendingFp> const reqs = std::move(ctx->pending_fp);
+	ctx->pending_fp.clear();
+	std::vector<std::unique_ptr<GenJob>> jobs;
 	DevBuf<gf2poly> d_local;
-	DevBuf<uint64_t> d_prefix, d_tiles, d_total;
-	SPICE_HIP(d_groups.alloc(ngroups));
 	SPICE_HIP(d_local.alloc(kGroup));
-	SPICE_HIP(d_block_state.alloc(P.nblk));
-	SPICE_HIP(d_prefix.alloc(P.nblk + 1));
-	int64_t const ntiles = (P.nblk + 1 + 4095) / 4096;
-	SPICE_HIP(d_tiles.alloc(ntiles));
-	SPICE_HIP(d_total.alloc(1));
-	SPICE_HIP(hipMemcpyAsync(d_groups.p, groups.data(), sizeof(xoro_state) * ngroups,
-	                         hipMemcpyHostToDevice, st));
-	SPICE_HIP(hipMemcpyAsync(d_local.p, local_polys().data(), sizeof(gf2poly) * kGroup,
-	                         hipMemcpyHostToDevice, st));
-	SPICE_HIP(hipMemsetAsync(d_prefix.p, 0, sizeof(uint64_t) * (P.nblk + 1), st));
-	{
-		int64_t const grid = std::min<int64_t>((P.nblk + 255) / 256, 65536);
-		hipLaunchKernelGGL(fp_draw_kernel, dim3(unsigned(grid)), dim3(256), 0, st, P, d_groups.p,
-		                   d_local.p, d_block_state.p, d_prefix.p);
-		SPICE_HIP(hipGetLastError());
-	}
-	int64_t const n = P.nblk + 1;  // includes a trailing zero: its prefix is the total
-	hipLaunchKernelGGL(scan_tile_sums, dim3(unsigned(ntiles)), dim3(256), 0, st, d_prefix.p, n, d_tiles.p);
-	hipLaunchKernelGGL(scan_tiles, dim3(1), dim3(256), 0, st, d_tiles.p, ntiles, d_total.p);
-	hipLaunchKernelGGL(scan_apply, dim3(unsigned(ntiles)), dim3(256), 0, st, d_prefix.p, n, d_tiles.p);
-	SPICE_HIP(hipGetLastError());
-
-	// phase B
-	DevBuf<int64_t> d_row_start;
-	DevBuf<ChainJob> d_job;
-	SPICE_HIP(d_row_start.alloc(S + 1));
-	SPICE_HIP(d_job.alloc(1));
-	ChainJob job{};
-	job.P            = P;
-	job.block_state  = d_block_state.p;
-	job.block_prefix = d_prefix.p;
-	job.row_start    = d_row_start.p;
-	job.r0           = 0;
-	job.s0           = 0;
-	int64_t host_rows = 0;
-	std::vector<int32_t> row;
-	for (;;) {
-		SPICE_HIP(hipMemcpyAsync(d_job.p, &job, sizeof(job), hipMemcpyHostToDevice, st));
-		hipLaunchKernelGGL(fp_chain_kernel, dim3(1), dim3(64), 0, st, d_job.p);
-		SPICE_HIP(hipGetLastError());
-		SPICE_HIP(hipMemcpyAsync(&job, d_job.p, sizeof(job), hipMemcpyDeviceToHost, st));
-		SPICE_HIP(hipStreamSynchronize(st));
-		if (job.status == 0) break;
-		if (job.status != 1) return set_error(SPICE_EDEVICE, "fixed_probability chain walk failed");
-		// the row's length depends on glibc's rounding: settle it with the C library
-		int64_t const r = job.status_row;
-		int64_t s       = 0;
-		SPICE_HIP(hipMemcpy(&s, d_row_start.p + r, sizeof(s), hipMemcpyDeviceToHost));
-		xoro_state bs;
-		SPICE_HIP(hipMemcpy(&bs, d_block_state.p + (s >> 6), sizeof(bs), hipMemcpyDeviceToHost));
-		for (int i = 0; i < int(s & 63); i++) xoro_next(bs);
-		int64_t const len = host_row(bs, P, row);
-		host_rows++;
-		job.r0     = r + 1;
-		job.s0     = s + len + 1;
-		job.status = 0;
-		if (r + 1 == S) {  // last row: record the end and finish
-			int64_t const end = job.s0;
-			SPICE_HIP(hipMemcpy(d_row_start.p + S, &end, sizeof(end), hipMemcpyHostToDevice));
-			break;
+	SPICE_HIP(hipMemcpyAsync(d_local.p, local_polys().data(), sizeof(gf2poly) * kGroup, hipMemcpyHostToDevice, st));
+	std::vector<std::vector<xoro_state>> group_host;
+	// phase A
+	for (auto const& q : reqs) {
+		auto J  = std::make_unique<GenJob>();
+		J->proj = q.proj;
+		J->S    = q.S;
+		J->D    = q.D;
+		J->seed = q.seed;
+		if (int r = fp_params(q.S, q.D, q.p, J->P, J->L)) return r;
+		FpParams const& P     = J->P;
+		int64_t const ngroups = (P.nblk + kGroup - 1) / kGroup;
+		group_host.emplace_back(ngroups);
+		auto& groups = group_host.back();
+		{
+			gf2poly const gstep = xoro_jump_poly(uint64_t(64) * kGroup);
+			xoro_state g        = q.seed;
+			for (int64_t i = 0; i < ngroups; i++) {
+				groups[i] = g;
+				g         = xoro_jump(g, gstep);
+			}
 		}
+		int64_t const ntiles = (P.nblk + 1 + 4095) / 4096;
+		SPICE_HIP(J->groups.alloc(ngroups));
+		SPICE_HIP(J->block_state.alloc(P.nblk));
+		SPICE_HIP(J->prefix.alloc(P.nblk + 1));
+		SPICE_HIP(J->tiles.alloc(ntiles));
+		SPICE_HIP(J->total.alloc(1));
+		SPICE_HIP(J->row_start.alloc(q.S + 1));
+		SPICE_HIP(hipMemcpyAsync(J->groups.p, groups.data(), sizeof(xoro_state) * ngroups, hipMemcpyHostToDevice, st));
+		SPICE_HIP(hipMemsetAsync(J->prefix.p, 0, sizeof(uint64_t) * (P.nblk + 1), st));
+		int64_t const grid = std::min<int64_t>((P.nblk + 255) / 256, 65536);
+		hipLaunchKernelGGL(fp_draw_kernel, dim3(unsigned(grid)), dim3(256), 0, st, P, J->groups.p, d_local.p,
+		                   J->block_state.p, J->prefix.p);
+		int64_t const n = P.nblk + 1;  // includes a trailing zero: its prefix is the total
+		hipLaunchKernelGGL(scan_tile_sums, dim3(unsigned(ntiles)), dim3(256), 0, st, J->prefix.p, n, J->tiles.p);
+		hipLaunchKernelGGL(scan_tiles, dim3(1), dim3(256), 0, st, J->tiles.p, ntiles, J->total.p);
+		hipLaunchKernelGGL(scan_apply, dim3(unsigned(ntiles)), dim3(256), 0, st, J->prefix.p, n, J->tiles.p);
+		SPICE_HIP(hipGetLastError());
+		J->job.P            = P;
+		J->job.block_state  = J->block_state.p;
+		J->job.block_prefix = J->prefix.p;
+		J->job.row_start    = J->row_start.p;
+		J->job.active       = 1;
+		jobs.push_back(std::move(J));
 	}
-	int64_t end = 0;
-	SPICE_HIP(hipMemcpy(&end, d_row_start.p + S, sizeof(end), hipMemcpyDeviceToHost));
-	pr.nnz = end - S;
-	if (pr.nnz < 0 || end > L) return set_error(SPICE_EDEVICE, "fixed_probability chain out of range");
-
+	// phase B: all chains at once; rows the device cannot settle are settled on
+	// the host (glibc's rounding) and the chains resumed
+	size_t const nj = jobs.size();
+	DevBuf<ChainJob> d_jobs;
+	SPICE_HIP(d_jobs.alloc(nj));
+	std::vector<ChainJob> cj(nj);
+	std::vector<int32_t> row;
+	for (size_t i = 0; i < nj; i++) cj[i] = jobs[i]->job;
+	for (;;) {
+		SPICE_HIP(hipMemcpyAsync(d_jobs.p, cj.data(), sizeof(ChainJob) * nj, hipMemcpyHostToDevice, st));
+		hipLaunchKernelGGL(fp_chain_kernel, dim3(unsigned(nj)), dim3(64), 0, st, d_jobs.p);
+		SPICE_HIP(hipGetLastError());
+		SPICE_HIP(hipMemcpyAsync(cj.data(), d_jobs.p, sizeof(ChainJob) * nj, hipMemcpyDeviceToHost, st));
+		SPICE_HIP(hipStreamSynchronize(st));
+		bool again = false;
+		for (size_t i = 0; i < nj; i++) {
+			ChainJob& job = cj[i];
+			if (!job.active) continue;
+			if (job.status == 0) {
+				job.active = 0;
+				continue;
+			}
+			if (job.status != 1) return set_error(SPICE_EDEVICE, "fixed_probability chain walk failed");
+			// the row's length depends on glibc's rounding: settle it with the C library
+			GenJob& J       = *jobs[i];
+			int64_t const r = job.status_row;
+			int64_t s       = 0;
+			SPICE_HIP(hipMemcpy(&s, J.row_start.p + r, sizeof(s), hipMemcpyDeviceToHost));
+			xoro_state bs;
+			SPICE_HIP(hipMemcpy(&bs, J.block_state.p + (s >> 6), sizeof(bs), hipMemcpyDeviceToHost));
+			for (int k = 0; k < int(s & 63); k++) xoro_next(bs);
+			int64_t const len = host_row(bs, J.P, row);
+			J.host_rows++;
+			job.r0     = r + 1;
+			job.s0     = s + len + 1;
+			job.status = 0;
+			if (r + 1 == J.S) {  // last row: record the end and finish
+				int64_t const end = job.s0;
+				SPICE_HIP(hipMemcpy(J.row_start.p + J.S, &end, sizeof(end), hipMemcpyHostToDevice));
+				job.active = 0;
+			} else {
+				again = true;
+			}
+		}
+		if (!again) break;
+	}
 	// phase C
-	SPICE_HIP(hipMalloc(reinterpret_cast<void**>(&pr.neighbors), sizeof(int32_t) * std::max<int64_t>(pr.nnz, 1)));
-	DevBuf<int64_t> d_unc;
-	DevBuf<int32_t> d_nunc;
-	DevBuf<unsigned long long> d_amb;
-	SPICE_HIP(d_unc.alloc(kMaxUncertain));
-	SPICE_HIP(d_nunc.alloc(1));
-	SPICE_HIP(d_amb.alloc(1));
-	SPICE_HIP(hipMemsetAsync(d_nunc.p, 0, sizeof(int32_t), st));
-	SPICE_HIP(hipMemsetAsync(d_amb.p, 0, sizeof(unsigned long long), st));
 	SPICE_HIP(hipMemsetAsync(ctx->error, 0, sizeof(int32_t), st));
-	{
-		int64_t const grid = std::min<int64_t>(S, 65536);
-		hipLaunchKernelGGL(fp_emit_kernel, dim3(unsigned(grid)), dim3(64), 0, st, P, d_block_state.p,
-		                   d_row_start.p, pr.offsets, pr.neighbors, d_unc.p, d_nunc.p, ctx->error,
-		                   d_amb.p);
+	for (size_t i = 0; i < nj; i++) {
+		GenJob& J = *jobs[i];
+		Proj& pr  = ctx->projs[J.proj];
+		int64_t end = 0;
+		SPICE_HIP(hipMemcpy(&end, J.row_start.p + J.S, sizeof(end), hipMemcpyDeviceToHost));
+		pr.nnz = end - J.S;
+		if (pr.nnz < 0 || end > J.L) return set_error(SPICE_EDEVICE, "fixed_probability chain out of range");
+		SPICE_HIP(hipMalloc(reinterpret_cast<void**>(&pr.neighbors), sizeof(int32_t) * std::max<int64_t>(pr.nnz, 1)));
+		SPICE_HIP(J.unc.alloc(kMaxUncertain));
+		SPICE_HIP(J.nunc.alloc(1));
+		SPICE_HIP(J.amb.alloc(1));
+		SPICE_HIP(hipMemsetAsync(J.nunc.p, 0, sizeof(int32_t), st));
+		SPICE_HIP(hipMemsetAsync(J.amb.p, 0, sizeof(unsigned long long), st));
+		int64_t const grid = std::min<int64_t>(J.S, 65536);
+		hipLaunchKernelGGL(fp_emit_kernel, dim3(unsigned(grid)), dim3(64), 0, st, J.P, J.block_state.p, J.row_start.p,
+		                   pr.offsets, pr.neighbors, J.unc.p, J.nunc.p, ctx->error, J.amb.p);
 		SPICE_HIP(hipGetLastError());
 	}
-	int32_t nunc = 0, err = 0;
-	unsigned long long amb = 0;
-	SPICE_HIP(hipMemcpyAsync(&nunc, d_nunc.p, sizeof(nunc), hipMemcpyDeviceToHost, st));
+	int32_t err = 0;
 	SPICE_HIP(hipMemcpyAsync(&err, ctx->error, sizeof(err), hipMemcpyDeviceToHost, st));
-	SPICE_HIP(hipMemcpyAsync(&amb, d_amb.p, sizeof(amb), hipMemcpyDeviceToHost, st));
 	SPICE_HIP(hipStreamSynchronize(st));
 	if (err) {
 		(void)hipMemset(ctx->error, 0, sizeof(int32_t));
 		return set_error(SPICE_EDEVICE, err == 3 ? "fixed_probability: row length disagreement"
 		                                         : "fixed_probability: too many undecided rows");
 	}
-	if (nunc > 0) {
-		std::vector<int64_t> rows(nunc), starts(2);
-		SPICE_HIP(hipMemcpy(rows.data(), d_unc.p, sizeof(int64_t) * nunc, hipMemcpyDeviceToHost));
-		for (int64_t r : rows) {
-			SPICE_HIP(hipMemcpy(starts.data(), d_row_start.p + r, sizeof(int64_t) * 2, hipMemcpyDeviceToHost));
-			xoro_state bs;
-			SPICE_HIP(hipMemcpy(&bs, d_block_state.p + (starts[0] >> 6), sizeof(bs), hipMemcpyDeviceToHost));
-			for (int i = 0; i < int(starts[0] & 63); i++) xoro_next(bs);
-			int64_t const len = host_row(bs, P, row);
-			if (len != starts[1] - starts[0] - 1)
-				return set_error(SPICE_EDEVICE, "fixed_probability: host row length disagreement");
-			if (len)
-				SPICE_HIP(hipMemcpy(pr.neighbors + (starts[0] - r), row.data(), sizeof(int32_t) * len,
-				                    hipMemcpyHostToDevice));
-			host_rows++;
+	for (size_t i = 0; i < nj; i++) {
+		GenJob& J = *jobs[i];
+		Proj& pr  = ctx->projs[J.proj];
+		int32_t nunc = 0;
+		unsigned long long amb = 0;
+		SPICE_HIP(hipMemcpy(&nunc, J.nunc.p, sizeof(nunc), hipMemcpyDeviceToHost));
+		SPICE_HIP(hipMemcpy(&amb, J.amb.p, sizeof(amb), hipMemcpyDeviceToHost));
+		if (nunc > 0) {
+			std::vector<int64_t> rows(nunc), starts(2);
+			SPICE_HIP(hipMemcpy(rows.data(), J.unc.p, sizeof(int64_t) * nunc, hipMemcpyDeviceToHost));
+			for (int64_t r : rows) {
+				SPICE_HIP(hipMemcpy(starts.data(), J.row_start.p + r, sizeof(int64_t) * 2, hipMemcpyDeviceToHost));
+				xoro_state bs;
+				SPICE_HIP(hipMemcpy(&bs, J.block_state.p + (starts[0] >> 6), sizeof(bs), hipMemcpyDeviceToHost));
+				for (int k = 0; k < int(starts[0] & 63); k++) xoro_next(bs);
+				int64_t const len = host_row(bs, J.P, row);
+				if (len != starts[1] - starts[0] - 1)
+					return set_error(SPICE_EDEVICE, "fixed_probability: host row length disagreement");
+				if (len)
+					SPICE_HIP(hipMemcpy(pr.neighbors + (starts[0] - r), row.data(), sizeof(int32_t) * len,
+					                    hipMemcpyHostToDevice));
+				J.host_rows++;
+			}
 		}
+		ctx->stats.csr_rows_host_resolved += J.host_rows;
+		ctx->stats.csr_ambiguous_draws += int64_t(amb + cj[i].ambiguous);
 	}
-	ctx->stats.csr_rows_host_resolved += host_rows;
-	ctx->stats.csr_ambiguous_draws += int64_t(amb + job.ambiguous);
 	ctx->stats.csr_ms +=
 	    std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_begin).count();
 	return SPICE_OK;

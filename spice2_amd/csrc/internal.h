@@ -76,6 +76,15 @@ struct Proj {
 	uint64_t* ages = nullptr;      // [src_size]
 };
 
+// A fixed_probability projection waiting for generation (projections are
+// generated together, see flush_fixed_probability).
+struct PendingFp {
+	int64_t proj;
+	int64_t S, D;
+	double p;
+	xoro_state seed;
+};
+
 // Device-side descriptors, rebuilt when populations/projections change.
 struct PopDev {
 	int64_t inst;
@@ -153,6 +162,7 @@ struct spice_ctx {
 	int64_t* stateful_dev = nullptr;        // ids of projections with per-synapse state
 	int64_t nstateful = 0;
 	bool any_plastic = false;
+	std::vector<spice_rt::PendingFp> pending_fp;  // projections to generate
 	int64_t max_pop_size = 0;
 	// the running window: device table [insts][SPICE_MAX_WINDOW] filled from one
 	// of two pinned staging buffers (the other may still be in flight)

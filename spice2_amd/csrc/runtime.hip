@@ -29,8 +29,10 @@ int set_error(int code, std::string const& msg) {
 	return code;
 }
 
-int generate_fixed_probability(spice_ctx* ctx, Proj& pr, int64_t src_size, int64_t dst_size,
-                               double p, xoro_state seed);  // csr_gen.hip
+// csr_gen.hip: fixed_probability projections are queued and generated together
+int queue_fixed_probability(spice_ctx* ctx, int64_t proj, int64_t src_size, int64_t dst_size, double p,
+                            xoro_state seed);
+int flush_fixed_probability(spice_ctx* ctx);
 int build_blocked(spice_ctx* ctx, Proj& Q, int64_t tile);  // blocked.hip
 }  // namespace spice_rt
 
@@ -420,6 +422,7 @@ void drop_graphs(spice_ctx* ctx) {
 }
 
 int refresh_device_descriptors(spice_ctx* ctx) {
+	if (int r = flush_fixed_probability(ctx)) return r;
 	if (!ctx->dev_dirty) return SPICE_OK;
 	drop_graphs(ctx);
 	SPICE_HIP(hipStreamSynchronize(ctx->stream));
@@ -797,6 +800,7 @@ void* spice_ctx_stream(spice_ctx* ctx) { return ctx->stream; }
 
 int spice_ctx_sync(spice_ctx* ctx) {
 	SPICE_HIP(hipSetDevice(ctx->device));
+	if (int r = flush_fixed_probability(ctx)) return r;
 	return check_device_error(ctx);
 }
 
@@ -844,6 +848,7 @@ int spice_instance_destroy(spice_ctx* ctx, int64_t inst) {
 	if (inst < 0 || inst >= int64_t(ctx->insts.size()) || !ctx->insts[inst].live)
 		return set_error(SPICE_EINVAL, "bad instance");
 	SPICE_HIP(hipSetDevice(ctx->device));
+	if (int r = flush_fixed_probability(ctx)) return r;
 	SPICE_HIP(hipStreamSynchronize(ctx->stream));
 	drop_graphs(ctx);
 	for (auto& Q : ctx->projs) {
@@ -1041,16 +1046,23 @@ int spice_proj_create_fixed_probability(spice_ctx* ctx, int64_t src, int64_t dst
 		(void)hipFree(Q.counts);
 		return precondition("topology.cpp", 106, "offsets.size() > src_count");
 	}
-	if (int r = generate_fixed_probability(ctx, Q, S, D, p, {seed_lo, seed_hi})) {
-		(void)hipFree(Q.counts);
+	int64_t const id = finish_proj(ctx, Q);
+	if (int r = queue_fixed_probability(ctx, id, S, D, p, {seed_lo, seed_hi})) {
+		Proj& bad = ctx->projs.back();
+		(void)hipFree(bad.counts);
+		(void)hipFree(bad.offsets);
+		ctx->pops[bad.dst].incoming.pop_back();
+		ctx->projs.pop_back();
 		return r;
 	}
-	*out_proj = finish_proj(ctx, Q);
+	*out_proj = id;
 	return SPICE_OK;
 }
 
 int spice_proj_synapses(spice_ctx* ctx, int64_t proj, int64_t* out) {
 	if (proj < 0 || proj >= int64_t(ctx->projs.size())) return set_error(SPICE_EINVAL, "bad projection");
+	SPICE_HIP(hipSetDevice(ctx->device));
+	if (int r = flush_fixed_probability(ctx)) return r;
 	*out = ctx->projs[proj].nnz;
 	return SPICE_OK;
 }
@@ -1059,6 +1071,7 @@ int spice_proj_download(spice_ctx* ctx, int64_t proj, int64_t* offsets, int32_t*
 	if (proj < 0 || proj >= int64_t(ctx->projs.size()) || ctx->projs[proj].dead)
 		return set_error(SPICE_EINVAL, "bad projection");
 	SPICE_HIP(hipSetDevice(ctx->device));
+	if (int r = flush_fixed_probability(ctx)) return r;
 	Proj& Q         = ctx->projs[proj];
 	int64_t const S = ctx->pops[Q.src].size;
 	SPICE_HIP(hipStreamSynchronize(ctx->stream));
@@ -1078,6 +1091,8 @@ fill_synapses_kernel(uint32_t* state, int64_t nnz, int32_t words, uint32_t const
 int spice_proj_set_synapses(spice_ctx* ctx, int64_t proj, void const* aos, int64_t count, int32_t struct_bytes) {
 	if (proj < 0 || proj >= int64_t(ctx->projs.size())) return set_error(SPICE_EINVAL, "bad projection");
 	if (struct_bytes <= 0 || struct_bytes > 256) return set_error(SPICE_EINVAL, "synapse struct of 1 to 256 bytes");
+	SPICE_HIP(hipSetDevice(ctx->device));
+	if (int r = flush_fixed_probability(ctx)) return r;
 	Proj& Q = ctx->projs[proj];
 	if (count != 1 && count != Q.nnz) return set_error(SPICE_EINVAL, "give one synapse struct or one per synapse");
 	if (Q.nnz >= (int64_t(1) << 32)) return set_error(SPICE_EINVAL, "stateful projections hold fewer than 2^32 synapses");
