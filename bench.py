@@ -77,9 +77,19 @@ def copy_calibration(reps: int = 5):
 
 
 def _pmc_rows(folder: Path, counter: str):
-    """(kernel name, value) per dispatch from rocprofv3 counter_collection CSVs."""
+    """(kernel name, value) per dispatch from a rocprofv3 --pmc output folder
+    (rocpd SQLite databases, or counter_collection CSVs)."""
     import csv
+    import sqlite3
     rows = []
+    for f in sorted(Path(folder).rglob("*results.db")):
+        con = sqlite3.connect(str(f))
+        try:
+            for k, n, v in con.execute("select kernel_name, counter_name, value from counters_collection"):
+                if n == counter:
+                    rows.append((k, float(v)))
+        finally:
+            con.close()
     for f in sorted(Path(folder).rglob("*counter_collection.csv")):
         with open(f, newline="") as fh:
             for r in csv.DictReader(fh):
@@ -244,7 +254,7 @@ def run_serve(n: int, rate: float, budget_gb: float, window: int, alone_sample: 
     kind, N, p, steps, arr, seeds = serve_requests(n, rate)
     fp = [S.job_footprint(int(k), int(x), float(q)) for k, x, q in zip(kind, N, p)]
     t0 = time.perf_counter()
-    adm, fin, res, ev = S.serve(kind, N, p, steps, arr, seeds, fp, int(budget_gb * 2**30), window)
+    adm, fin, res, ev, ph = S.serve(kind, N, p, steps, arr, seeds, fp, int(budget_gb * 2**30), window)
     wall = time.perf_counter() - t0
     lat = fin - arr
     sample = np.linspace(0, n - 1, alone_sample).round().astype(int)
@@ -254,6 +264,7 @@ def run_serve(n: int, rate: float, budget_gb: float, window: int, alone_sample: 
             "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
             "queue_wait_p50_s": float(np.percentile(adm - arr, 50)),
             "events": int(ev), "makespan_s": float(fin.max()), "wall_s": wall,
+            "host_seconds": {"build": ph[0], "rounds": ph[1], "retire": ph[2]},
             "value": ev / float(fin.max()), "unit": "synaptic events/s",
             "kinds": [int((kind == k).sum()) for k in range(3)],
             "sample_alone": len(sample), "sample_results_identical": bool(same),

@@ -88,8 +88,9 @@ struct engine {
 
 	int64_t add_instance(Int max_delay) {
 		int64_t id = 0;
-		check_spice(spice_instance_create(ctx, max_delay, &id));
-		pending.resize(size_t(id) + 1);
+		check_spice(spice_instance_create(ctx, max_delay, &id));  // may reuse a destroyed instance's id
+		if (pending.size() <= size_t(id)) pending.resize(size_t(id) + 1);
+		pending[size_t(id)].clear();
 		return id;
 	}
 

@@ -80,10 +80,12 @@ public:
 			// admit every job that has arrived and fits
 			while (next < _queue.size() && _queue[order[next]].arrival <= now() &&
 			       (act.empty() || used + _queue[order[next]].footprint <= _budget)) {
-				request& r = _queue[order[next]];
-				batch* b   = r.single_step ? _single.get() : _multi.get();
-				snn& net   = b->add(r.dt, r.max_delay, r.seed);
+				request& r    = _queue[order[next]];
+				batch* b      = r.single_step ? _single.get() : _multi.get();
+				double const tb = now();
+				snn& net      = b->add(r.dt, r.max_delay, r.seed);
 				r.build(net);
+				_t_build += now() - tb;
 				used += r.footprint;
 				out[order[next]] = {r.arrival, now(), 0.0, r.steps};
 				act.push_back({order[next], &net, b, r.steps});
@@ -96,6 +98,7 @@ public:
 			}
 			// one round: every admitted job queues up to `window` steps; both
 			// classes run their windows (asynchronously, on their own streams)
+			double const tr = now();
 			for (auto& a : act) {
 				Int const n = std::min(a.left, _window);
 				for (Int k = 0; k < n; k++) a.net->step();
@@ -103,6 +106,8 @@ public:
 			}
 			_multi->flush();
 			_single->flush();
+			_t_round += now() - tr;
+			double const tt = now();
 			// retire finished jobs
 			for (size_t i = 0; i < act.size();) {
 				if (act[i].left > 0) {
@@ -116,6 +121,7 @@ public:
 				used -= r.footprint;
 				act.erase(act.begin() + long(i));
 			}
+			_t_retire += now() - tt;
 		}
 		_queue.clear();
 		return out;
@@ -124,9 +130,16 @@ public:
 	// Synaptic events delivered by all jobs so far.
 	int64_t events() { return events(*_multi) + events(*_single); }
 
+	// Host seconds spent building admitted networks, launching rounds (including
+	// waits on the device), and reading and freeing finished jobs.
+	double build_seconds() const { return _t_build; }
+	double round_seconds() const { return _t_round; }
+	double retire_seconds() const { return _t_retire; }
+
 private:
 	int64_t _budget;
 	Int _window;
+	double _t_build = 0, _t_round = 0, _t_retire = 0;
 	std::unique_ptr<batch> _multi, _single;
 	std::vector<request> _queue;
 

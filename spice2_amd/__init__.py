@@ -212,7 +212,7 @@ def workloads() -> C.CDLL:
         _dp = C.POINTER(C.c_double)
         _ip = C.POINTER(_i64)
         L.wl_serve_run.argtypes = [C.c_int, C.POINTER(C.c_int), _ip, _dp, _ip, _dp, C.POINTER(C.c_uint32), _ip,
-                                   _i64, _i64, _dp, _dp, C.POINTER(_u64), _ip]
+                                   _i64, _i64, _dp, _dp, C.POINTER(_u64), _ip, _dp]
         L.wl_job_alone.argtypes = [C.c_int, _i64, C.c_double, _i64, C.POINTER(C.c_uint32), C.POINTER(_u64)]
         _wl = L
     return _wl
@@ -335,7 +335,8 @@ def job_footprint(kind: int, N: int, p: float, tile: int = 8192) -> int:
 
 def serve(kind, N, p, steps, arrival, seeds, footprint, budget, window=100):
     """Runs jobs through spice::server (libspice_workloads.so, wl_serve_run).
-    Returns (admitted, finished, result hashes, synaptic events)."""
+    Returns (admitted, finished, result hashes, synaptic events, host seconds
+    spent building / running rounds / retiring)."""
     L = workloads()
     n = len(kind)
     arr = lambda t, xs: (t * n)(*xs)
@@ -343,14 +344,15 @@ def serve(kind, N, p, steps, arrival, seeds, footprint, budget, window=100):
     fin = (C.c_double * n)()
     res = (_u64 * n)()
     ev = _i64()
+    ph = (C.c_double * 3)()
     sw = (C.c_uint32 * (2 * n))(*[int(w) for s in seeds for w in s])
     r = L.wl_serve_run(n, arr(C.c_int, [int(x) for x in kind]), arr(_i64, [int(x) for x in N]),
                        arr(C.c_double, [float(x) for x in p]), arr(_i64, [int(x) for x in steps]),
                        arr(C.c_double, [float(x) for x in arrival]), sw, arr(_i64, [int(x) for x in footprint]),
-                       int(budget), int(window), adm, fin, res, C.byref(ev))
+                       int(budget), int(window), adm, fin, res, C.byref(ev), ph)
     if r != 0:
         raise SpiceError(L.wl_error().decode())
-    return np.array(adm[:]), np.array(fin[:]), [int(x) for x in res], ev.value
+    return np.array(adm[:]), np.array(fin[:]), [int(x) for x in res], ev.value, list(ph)
 
 
 def job_alone(kind, N, p, steps, seed):

@@ -466,7 +466,8 @@ float job_delay(int kind) { return kind == 1 ? 1e-4f : 15e-4f; }
 // admitted/finished times (s) and the result hash; *events = synaptic events.
 int wl_serve_run(int n, int const* kind, int64_t const* N, double const* p, int64_t const* steps,
                  double const* arrival, uint32_t const* seeds, int64_t const* footprint, int64_t budget,
-                 int64_t window, double* admitted, double* finished, uint64_t* result, int64_t* events) {
+                 int64_t window, double* admitted, double* finished, uint64_t* result, int64_t* events,
+                 double* phase_seconds) {
 	try {
 		spice::server srv(budget, window, 15);
 		for (int i = 0; i < n; i++) {
@@ -490,7 +491,10 @@ int wl_serve_run(int n, int const* kind, int64_t const* N, double const* p, int6
 			admitted[i] = recs[size_t(i)].admitted;
 			finished[i] = recs[size_t(i)].finished;
 		}
-		*events = srv.events();
+		*events          = srv.events();
+		phase_seconds[0] = srv.build_seconds();
+		phase_seconds[1] = srv.round_seconds();
+		phase_seconds[2] = srv.retire_seconds();
 		return 0;
 	} catch (std::exception const& e) {
 		return fail(e);

@@ -23,7 +23,7 @@ def test_served_jobs_equal_jobs_alone():
     fp = [S.job_footprint(k, n, q) for k, n, q in zip(kind, N, p)]
     budget = 2 * max(fp)  # forces later jobs to wait for memory
     print("serving", len(JOBS), "jobs, budget", budget, flush=True)
-    adm, fin, res, ev = S.serve(kind, N, p, steps, arr, seeds, fp, budget, window=50)
+    adm, fin, res, ev, _ = S.serve(kind, N, p, steps, arr, seeds, fp, budget, window=50)
     assert ev > 0
     assert (adm >= np.array(arr) - 1e-9).all() and (fin > adm).all()
     for i, (k, n, q, st, _, sd) in enumerate(JOBS):
