@@ -515,12 +515,15 @@ __device__ inline void catch_up_row(Syn const& f, spice_proj_view const& P, int6
 }
 
 // The sweep at steps that are multiples of 64 (snn.cpp:17-19): every source.
+// blockIdx.y: the projection (projs[y], functor funcs[y]) of a group of plastic
+// projections with one synapse type.
 template <class Syn>
-__global__ void __launch_bounds__(256) plastic_sweep_kernel(spice_block b, int64_t proj, Syn const* fobj) {
-	spice_proj_view const P = b.projs[proj];
+__global__ void __launch_bounds__(256) plastic_sweep_kernel(spice_block b, int64_t const* projs, Syn const* funcs) {
+	spice_proj_view const P = b.projs[projs[blockIdx.y]];
+	if (P.inst < 0) return;
 	spice_step const st     = step_of(b, P.inst, 0);
 	if (st.local < 0 || st.local % 64 != 0) return;
-	Syn const f       = cload_obj<Syn>(fobj);
+	Syn const f       = cload_obj<Syn>(funcs + blockIdx.y);
 	int const lane    = threadIdx.x % 64;
 	int64_t const gw  = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) / 64;
 	int64_t const nw  = int64_t(gridDim.x) * blockDim.x / 64;
@@ -532,12 +535,14 @@ __global__ void __launch_bounds__(256) plastic_sweep_kernel(spice_block b, int64
 // delay - 1 steps ago are brought up to the step; their events were listed by
 // the runtime and are applied by the targets at the next step.
 template <class Syn>
-__global__ void __launch_bounds__(256) plastic_deliver_kernel(spice_block b, int64_t proj, Syn const* fobj) {
-	spice_proj_view const P = b.projs[proj];
+__global__ void __launch_bounds__(256) plastic_deliver_kernel(spice_block b, int64_t const* projs,
+                                                              Syn const* funcs) {
+	spice_proj_view const P = b.projs[projs[blockIdx.y]];
+	if (P.inst < 0) return;
 	spice_step const st     = step_of(b, P.inst, 0);
 	int64_t const t         = st.local;
 	if (t < 0 || t < P.delay - 1) return;
-	Syn const f          = cload_obj<Syn>(fobj);
+	Syn const f          = cload_obj<Syn>(funcs + blockIdx.y);
 	int64_t const slot   = (t - P.delay + 1) % P.ring;
 	int64_t const nsp    = P.src_ring_cnt[slot];
 	int32_t const* ids   = P.src_ring_ids + slot * P.src_size;

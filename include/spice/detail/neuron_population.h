@@ -62,6 +62,15 @@ struct group_base {
 	virtual void launch(spice_block const& b, hipStream_t s)     = 0;
 };
 
+// The plastic projections of one synapse type (in every instance): one sweep
+// and one delivery launch per step cover all of them.
+struct plastic_group_base {
+	virtual ~plastic_group_base()                                 = default;
+	virtual void prepare()                                        = 0;  // device tables, before a window
+	virtual void launch(spice_block const& b, hipStream_t s)      = 0;
+	virtual void remove_instance(int64_t inst)                    = 0;
+};
+
 // Owns the device context.  Steps are queued per instance (snn::step()) and
 // run in windows when results are needed or a window is full.
 struct engine {
@@ -74,9 +83,8 @@ struct engine {
 	Int window    = SPICE_MAX_WINDOW;  // steps per spice_run() (graphs are cached per length)
 	bool hold     = false;             // a batch queues whole rounds before flushing
 	std::string cb_error;
-	// per plastic projection (with its instance): its sweep and delivery kernels
-	// (synapse phase)
-	std::vector<std::pair<int64_t, std::function<void(spice_block const&, hipStream_t)>>> synapse_launchers;
+	// plastic projections by synapse type (synapse phase)
+	std::vector<std::pair<std::type_index, std::unique_ptr<plastic_group_base>>> plastic_groups;
 
 	explicit engine(Int max_delay) {
 		check_spice(spice_ctx_create(0, max_delay, &ctx));
@@ -122,7 +130,7 @@ struct engine {
 			if (b->phase == SPICE_PHASE_NEURONS)
 				for (auto& g : e->groups) g.second->launch(*b, static_cast<hipStream_t>(stream));
 			else
-				for (auto& l : e->synapse_launchers) l.second(*b, static_cast<hipStream_t>(stream));
+				for (auto& g : e->plastic_groups) g.second->launch(*b, static_cast<hipStream_t>(stream));
 		} catch (std::exception const& x) {
 			e->cb_error = x.what();
 			return 1;
@@ -150,7 +158,7 @@ struct NeuronPopulation {
 inline void engine::remove_instance(int64_t inst) noexcept {
 	if (size_t(inst) < pending.size()) pending[size_t(inst)].clear();
 	std::erase_if(pops, [&](NeuronPopulation* p) { return p->instance() == inst; });
-	std::erase_if(synapse_launchers, [&](auto const& l) { return l.first == inst; });
+	for (auto& g : plastic_groups) g.second->remove_instance(inst);
 	groups.clear();
 	regroup = true;
 	(void)spice_instance_destroy(ctx, inst);
@@ -160,6 +168,7 @@ inline void engine::flush() {
 	size_t const most = most_pending();
 	if (most == 0) return;
 	for (auto* p : pops) p->prepare();
+	for (auto& g : plastic_groups) g.second->prepare();
 	if (regroup) {
 		groups.clear();
 		for (auto* p : pops) {

@@ -3,8 +3,10 @@
 // functor is replayed by the target population's update kernel.
 #pragma once
 
+#include <algorithm>
 #include <span>
 #include <type_traits>
+#include <typeindex>
 #include <vector>
 
 #include "spice/concepts.h"
@@ -16,6 +18,55 @@
 #include "spice_hip.h"
 
 namespace spice::detail {
+template <class Syn>
+struct plastic_group final : plastic_group_base {
+	struct member {
+		int64_t inst, proj, rows;
+		Syn syn;
+	};
+	std::vector<member> members;
+	dev_array<int64_t> ids;
+	dev_array<Syn> funcs;  // whole 16-byte units: the kernels read functors word by word
+	int64_t most_rows = 0;
+	bool dirty        = true;
+
+	void prepare() override {
+		if (!dirty) return;
+		std::vector<int64_t> h;
+		std::vector<Syn> f;
+		most_rows = 0;
+		for (auto& m : members) {
+			h.push_back(m.proj);
+			f.push_back(m.syn);
+			most_rows = std::max(most_rows, m.rows);
+		}
+		f.resize(f.size() + 16 / sizeof(Syn) + 1, members.empty() ? Syn{} : members.back().syn);
+		ids.assign(h);
+		funcs.assign(f);
+		dirty = false;
+	}
+	void launch(spice_block const& b, hipStream_t st) override {
+		if (members.empty()) return;
+		unsigned const n     = unsigned(members.size());
+		unsigned const sweep = unsigned(std::clamp<int64_t>((most_rows + 3) / 4, 1, 4096));
+		hipLaunchKernelGGL((plastic_sweep_kernel<Syn>), dim3(sweep, n), dim3(256), 0, st, b, ids.p, funcs.p);
+		hipLaunchKernelGGL((plastic_deliver_kernel<Syn>), dim3(64, n), dim3(256), 0, st, b, ids.p, funcs.p);
+		check_hip(hipGetLastError(), "plastic synapse launch");
+	}
+	void remove_instance(int64_t inst) override {
+		if (std::erase_if(members, [&](member const& m) { return m.inst == inst; })) dirty = true;
+	}
+};
+
+template <class Syn>
+plastic_group<Syn>& plastic_group_of(engine* eng) {
+	auto const key = std::type_index(typeid(plastic_group<Syn>));
+	for (auto& g : eng->plastic_groups)
+		if (g.first == key) return static_cast<plastic_group<Syn>&>(*g.second);
+	eng->plastic_groups.emplace_back(key, std::make_unique<plastic_group<Syn>>());
+	return static_cast<plastic_group<Syn>&>(*eng->plastic_groups.back().second);
+}
+
 struct SynapsePopulation {
 	virtual ~SynapsePopulation() = default;
 	virtual Int delay() const    = 0;
@@ -55,25 +106,15 @@ public:
 		if constexpr (PlasticSynapse<Syn>) {
 			check_spice(spice_proj_set_plastic(eng->ctx, _id));
 			src->plastic();
-			check_hip(hipMalloc(&_fobj, (sizeof(Syn) + 15) / 16 * 16), "hipMalloc");
-			check_hip(hipMemcpy(_fobj, &_syn, sizeof(Syn), hipMemcpyHostToDevice), "hipMemcpy");
-			int64_t const id   = _id;
-			Syn const* fobj    = static_cast<Syn const*>(_fobj);
-			int64_t const rows = src->size();
-			eng->synapse_launchers.emplace_back(inst, [id, fobj, rows](spice_block const& b, hipStream_t st) {
-				unsigned const sweep = unsigned(std::clamp<int64_t>((rows + 3) / 4, 1, 4096));
-				hipLaunchKernelGGL((plastic_sweep_kernel<Syn>), dim3(sweep), dim3(256), 0, st, b, id, fobj);
-				hipLaunchKernelGGL((plastic_deliver_kernel<Syn>), dim3(256), dim3(256), 0, st, b, id, fobj);
-				check_hip(hipGetLastError(), "plastic synapse launch");
-			});
+			auto& g = plastic_group_of<Syn>(eng);
+			g.members.push_back({inst, _id, src->size(), _syn});
+			g.dirty = true;
 		}
 		dst->template add_incoming<Syn>(_syn);
 		eng->min_delay = std::min(eng->min_delay, delay);
 	}
 
-	~synapse_population() override {
-		if (_fobj) (void)hipFree(_fobj);
-	}
+
 
 	Int delay() const override { return _delay; }
 	int64_t id() const override { return _id; }
@@ -81,8 +122,7 @@ public:
 private:
 	Syn _syn;
 	Int _delay;
-	int64_t _id  = -1;
-	void* _fobj = nullptr;  // device copy of the functor (plastic kernels)
+	int64_t _id = -1;
 
 	// Every synapse starts as a default synapse; a per-synapse init() runs on
 	// the host in CSR order with its own generator (synapse_population.h:34-40).

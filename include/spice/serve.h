@@ -53,6 +53,9 @@ public:
 	_multi(std::make_unique<batch>(max_delay_steps)), _single(std::make_unique<batch>(max_delay_steps)) {
 		_multi->set_window(window);
 		_single->set_window(window);
+		// rounds between admissions and retirements replay the window's launches
+		_multi->set_graphs(true);
+		_single->set_graphs(true);
 	}
 
 	void submit(request r) { _queue.push_back(std::move(r)); }

@@ -68,9 +68,13 @@ __device__ __forceinline__ uint64_t quantize(double e, double two_F) {
 	return uint64_t(e * two_F + 0.5);
 }
 
-__device__ __forceinline__ xoro_state state_at(xoro_state const* block_state, int64_t pos) {
-	xoro_state st = block_state[pos >> 6];
-	for (int i = 0, n = int(pos & 63); i < n; i++) xoro_next(st);
+// Generator states are kept every kSub positions (8 per block of 64), so any
+// position is at most kSub - 1 steps away from a stored state.
+constexpr int kSub = 8;
+
+__device__ __forceinline__ xoro_state state_at(xoro_state const* sub_state, int64_t pos) {
+	xoro_state st = sub_state[pos / kSub];
+	for (int i = 0, n = int(pos % kSub); i < n; i++) xoro_next(st);
 	return st;
 }
 
@@ -78,16 +82,19 @@ __device__ __forceinline__ double wave_bcast(double v, int lane) {
 	return __shfl(v, lane, kWave);
 }
 
-// Phase A: block states and per-block fixed-point draw sums.
+// Phase A: generator states (every kSub positions) and per-block fixed-point
+// draw sums.
 __global__ void __launch_bounds__(256)
 fp_draw_kernel(FpParams P, xoro_state const* group_state, gf2poly const* local_poly,
-               xoro_state* block_state, uint64_t* block_sum) {
+               xoro_state* sub_state, uint64_t* block_sum) {
 	int64_t const stride = int64_t(gridDim.x) * blockDim.x;
 	for (int64_t b = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; b < P.nblk; b += stride) {
-		xoro_state st  = xoro_jump(group_state[b / kGroup], local_poly[b % kGroup]);
-		block_state[b] = st;
-		uint64_t sum   = 0;
-		for (int i = 0; i < 64; i++) sum += quantize(fast_draw(xoro_next(st), P.scale), P.two_F);
+		xoro_state st = xoro_jump(group_state[b / kGroup], local_poly[b % kGroup]);
+		uint64_t sum  = 0;
+		for (int i = 0; i < 64; i++) {
+			if (i % kSub == 0) sub_state[b * (64 / kSub) + i / kSub] = st;
+			sum += quantize(fast_draw(xoro_next(st), P.scale), P.two_F);
+		}
 		block_sum[b] = sum;
 	}
 }
@@ -166,7 +173,7 @@ __device__ RowResult row_exact(FpParams const& P, xoro_state const* block_state,
 // Phase B: walk the row chain.  One 64-thread workgroup per projection.
 struct ChainJob {
 	FpParams P;
-	xoro_state const* block_state;
+	xoro_state const* block_state;  // generator state every kSub positions
 	uint64_t const* block_prefix;  // nblk + 1 entries
 	int64_t* row_start;            // S + 1 entries
 	int64_t r0, s0;                // resume point
@@ -192,12 +199,44 @@ __device__ uint64_t prefix_at(FpParams const& P, xoro_state const* block_state,
 	return block_prefix[b] + q;
 }
 
+// Blocks of the prefix and state tables the walk keeps in LDS: a row spans
+// about (M + 1) / 64 blocks, so a window serves many consecutive rows and the
+// walk reads global memory once per window instead of twice per row.
+constexpr int kChainWin = 256;
+
 __global__ void __launch_bounds__(64) fp_chain_kernel(ChainJob* jobs) {
 	__shared__ double lds_lo[1024], lds_hi[1024];
+	__shared__ uint64_t wpre[kChainWin], ws0[kChainWin * 8], ws1[kChainWin * 8];
 	ChainJob& J      = jobs[blockIdx.x];
 	if (!J.active) return;
 	FpParams const P = J.P;
 	int const lane   = threadIdx.x;
+	int64_t wb = 0, wn = 0;  // window: blocks [wb, wb + wn)
+	bool win   = false;      // the current row's blocks are in the window
+	// makes blocks [lo, hi] resident if they fit (uniform across the wave)
+	auto stage = [&](int64_t lo, int64_t hi) {
+		if (hi - lo + 1 > kChainWin) {
+			win = false;
+			return;
+		}
+		win = true;
+		if (wn > 0 && lo >= wb && hi < wb + wn) return;
+		__syncthreads();
+		wb = lo;
+		wn = min<int64_t>(kChainWin, P.nblk + 1 - lo);
+		for (int64_t i = lane; i < wn; i += kWave) wpre[i] = J.block_prefix[wb + i];
+		int64_t const ns = min<int64_t>(wn, P.nblk - wb) * (64 / kSub);
+		for (int64_t i = lane; i < ns; i += kWave) {
+			xoro_state const st = J.block_state[wb * (64 / kSub) + i];
+			ws0[i]              = st.s0;
+			ws1[i]              = st.s1;
+		}
+		__syncthreads();
+	};
+	auto pre = [&](int64_t b) -> uint64_t { return win ? wpre[b - wb] : J.block_prefix[b]; };
+	auto sst = [&](int64_t q) -> xoro_state {  // state at position kSub * q
+		return win ? xoro_state{ws0[q - wb * (64 / kSub)], ws1[q - wb * (64 / kSub)]} : J.block_state[q];
+	};
 	int64_t s        = J.s0;
 	uint64_t Cs      = prefix_at(P, J.block_state, J.block_prefix, s);
 	unsigned long long amb = 0;
@@ -207,12 +246,13 @@ __global__ void __launch_bounds__(64) fp_chain_kernel(ChainJob* jobs) {
 		int64_t const last = s + P.M + 1;  // cap position for m
 		// coarse: block boundaries q in [s+1, last]
 		int64_t const b_lo = (s >> 6) + 1, b_hi = last >> 6;
+		stage(s >> 6, min<int64_t>(b_hi, P.nblk - 1));
 		int64_t fb = -1;  // block holding m - 1
 		for (int64_t b0 = b_lo; b0 <= b_hi && fb < 0; b0 += kWave) {
 			int64_t const b = b0 + lane;
 			bool hit        = false;
 			if (b <= b_hi) {
-				uint64_t const G = (J.block_prefix[b] - Cs) + (uint64_t((b << 6) - 1 - s) << P.F);
+				uint64_t const G = (pre(b) - Cs) + (uint64_t((b << 6) - 1 - s) << P.F);
 				hit              = G >= P.theta;
 			}
 			uint64_t const m = __ballot(hit);
@@ -223,15 +263,16 @@ __global__ void __launch_bounds__(64) fp_chain_kernel(ChainJob* jobs) {
 		int64_t const pj = (fb << 6) + lane;
 		uint64_t qv      = 0;
 		{
-			xoro_state st = state_at(J.block_state, pj);
-			qv            = quantize(fast_draw(xoro_next(st), P.scale), P.two_F);
+			xoro_state st = sst(pj / kSub);
+			for (int i = 0; i < int(pj % kSub); i++) xoro_next(st);
+			qv = quantize(fast_draw(xoro_next(st), P.scale), P.two_F);
 		}
 		uint64_t incl = qv;  // inclusive wave scan
 		for (int off = 1; off < kWave; off <<= 1) {
 			uint64_t const t = __shfl_up(incl, off, kWave);
 			if (lane >= off) incl += t;
 		}
-		uint64_t const Cnext = J.block_prefix[fb] + incl;  // C(pj + 1)
+		uint64_t const Cnext = pre(fb) + incl;  // C(pj + 1)
 		bool const valid     = pj >= s && pj <= last - 1;
 		uint64_t const G     = (Cnext - Cs) + (uint64_t(pj - s) << P.F);
 		uint64_t const cross = __ballot(valid && G >= P.theta);
@@ -254,7 +295,7 @@ __global__ void __launch_bounds__(64) fp_chain_kernel(ChainJob* jobs) {
 			if (jl > 0)
 				gp = G_pm;
 			else  // m - 1 is the first position of block fb: boundary value
-				gp = (J.block_prefix[fb] - Cs) + (uint64_t((fb << 6) - 1 - s) << P.F);
+				gp = (pre(fb) - Cs) + (uint64_t((fb << 6) - 1 - s) << P.F);
 			if (!(gp < P.theta && P.theta - gp >= P.tol)) uncertain = true;
 		}
 		uint64_t Cm = __shfl(Cnext, jl, kWave);
@@ -471,7 +512,7 @@ int flush_fixed_probability(spice_ctx* ctx) {
 		}
 		int64_t const ntiles = (P.nblk + 1 + 4095) / 4096;
 		SPICE_HIP(J->groups.alloc(ngroups));
-		SPICE_HIP(J->block_state.alloc(P.nblk));
+		SPICE_HIP(J->block_state.alloc(P.nblk * (64 / kSub)));
 		SPICE_HIP(J->prefix.alloc(P.nblk + 1));
 		SPICE_HIP(J->tiles.alloc(ntiles));
 		SPICE_HIP(J->total.alloc(1));
@@ -493,14 +534,14 @@ int flush_fixed_probability(spice_ctx* ctx) {
 		J->job.active       = 1;
 		jobs.push_back(std::move(J));
 	}
-	// phase B: all chains at once; rows the device cannot settle are settled on
-	// the host (glibc's rounding) and the chains resumed
+	// phase B: all chains at once (one workgroup each); rows the device cannot
+	// settle are settled on the host (glibc's rounding) and the chains resumed
 	size_t const nj = jobs.size();
-	DevBuf<ChainJob> d_jobs;
-	SPICE_HIP(d_jobs.alloc(nj));
 	std::vector<ChainJob> cj(nj);
 	std::vector<int32_t> row;
 	for (size_t i = 0; i < nj; i++) cj[i] = jobs[i]->job;
+	DevBuf<ChainJob> d_jobs;
+	SPICE_HIP(d_jobs.alloc(nj));
 	for (;;) {
 		SPICE_HIP(hipMemcpyAsync(d_jobs.p, cj.data(), sizeof(ChainJob) * nj, hipMemcpyHostToDevice, st));
 		hipLaunchKernelGGL(fp_chain_kernel, dim3(unsigned(nj)), dim3(64), 0, st, d_jobs.p);
@@ -522,8 +563,8 @@ int flush_fixed_probability(spice_ctx* ctx) {
 			int64_t s       = 0;
 			SPICE_HIP(hipMemcpy(&s, J.row_start.p + r, sizeof(s), hipMemcpyDeviceToHost));
 			xoro_state bs;
-			SPICE_HIP(hipMemcpy(&bs, J.block_state.p + (s >> 6), sizeof(bs), hipMemcpyDeviceToHost));
-			for (int k = 0; k < int(s & 63); k++) xoro_next(bs);
+			SPICE_HIP(hipMemcpy(&bs, J.block_state.p + s / kSub, sizeof(bs), hipMemcpyDeviceToHost));
+			for (int k = 0; k < int(s % kSub); k++) xoro_next(bs);
 			int64_t const len = host_row(bs, J.P, row);
 			J.host_rows++;
 			job.r0     = r + 1;
@@ -580,8 +621,8 @@ int flush_fixed_probability(spice_ctx* ctx) {
 			for (int64_t r : rows) {
 				SPICE_HIP(hipMemcpy(starts.data(), J.row_start.p + r, sizeof(int64_t) * 2, hipMemcpyDeviceToHost));
 				xoro_state bs;
-				SPICE_HIP(hipMemcpy(&bs, J.block_state.p + (starts[0] >> 6), sizeof(bs), hipMemcpyDeviceToHost));
-				for (int k = 0; k < int(starts[0] & 63); k++) xoro_next(bs);
+				SPICE_HIP(hipMemcpy(&bs, J.block_state.p + starts[0] / kSub, sizeof(bs), hipMemcpyDeviceToHost));
+				for (int k = 0; k < int(starts[0] % kSub); k++) xoro_next(bs);
 				int64_t const len = host_row(bs, J.P, row);
 				if (len != starts[1] - starts[0] - 1)
 					return set_error(SPICE_EDEVICE, "fixed_probability: host row length disagreement");
