@@ -161,7 +161,8 @@ int spice_ctx_create(int device, int64_t max_delay, spice_ctx** out);
 void spice_ctx_destroy(spice_ctx* ctx);
 /* Delivery algorithm (default SPICE_DELIVER_PULL); fixed once steps have run. */
 int spice_ctx_set_delivery(spice_ctx* ctx, int mode);
-/* Targets per pull tile (multiple of 64, <= 65536; LDS use is 2 bytes per target). */
+/* Targets per pull tile (multiple of 64, <= 32768; LDS use is 4 bytes per target and
+ * projection counted together). */
 int spice_ctx_set_tile(spice_ctx* ctx, int64_t tile);
 int64_t spice_ctx_tile(spice_ctx* ctx);
 /* Pull tile of one target population (0: the context's); exact for any tile. */

@@ -16,6 +16,10 @@ namespace spice_rt {
 // Sets the thread's last error and returns `code`.
 int set_error(int code, std::string const& msg);
 
+// Largest pull tile: one projection's 32-bit counters for the tile must fit in
+// the LDS of a compute unit beside the run descriptors.
+constexpr int64_t kMaxPullTile = 32768;
+
 #define SPICE_HIP(expr)                                                                    \
 	do {                                                                                   \
 		hipError_t const e_ = (expr);                                                      \
@@ -61,11 +65,13 @@ struct Proj {
 	uint32_t* counts = nullptr;    // [slots][dst_size]
 	int64_t slots = 0;
 	// Tile view for pull delivery (blocked.hip): 16-bit tile-local targets in
-	// CSR order and, per row, the in-row start of each tile's run.
+	// runs padded to whole 16-byte vectors and, per row, the first vector of
+	// each tile's run.
 	int64_t tile = 0;              // 0: not built
 	int64_t ntile = 0;
-	uint16_t* rt = nullptr;        // [src_size][ntile + 1]
-	uint16_t* btgt = nullptr;      // [nnz], padded to whole 16-byte vectors
+	int64_t nvec = 0;              // 16-byte vectors of btgt (one spare vector follows)
+	uint32_t* rv = nullptr;        // [src_size][ntile + 1]
+	uint16_t* btgt = nullptr;      // [nvec + 1][8]
 	// per-synapse state and event lists (stateful synapses)
 	int32_t syn_words = 0;
 	uint32_t* syn_state = nullptr; // [syn_words][nnz]
@@ -107,7 +113,7 @@ struct ProjDev {
 	int32_t const* src_ring_ids;
 	int32_t const* src_ring_cnt;
 	int64_t src_size;
-	uint16_t const* rt;
+	uint32_t const* rv;
 	uint16_t const* btgt;
 	int64_t ntile;
 	uint32_t* ev_offsets;

@@ -159,26 +159,29 @@ __global__ void __launch_bounds__(256) deliver_push_kernel(ProjDev const* projs,
 }
 
 // Pull delivery: one workgroup per (target tile, step).  The workgroup counts,
-// in LDS, the events that the step's spikes of every incoming projection put
-// on each target of its tile, then writes the tile's counters once: no global
-// atomics.  Counters are 16 bits, two per LDS word; a target receives at most
-// one event per spiking source of a projection, so they cannot overflow while
-// fewer than 65536 sources of one projection spike - larger spike sets are
-// processed in chunks of 65535 whose counts are added into the (workgroup-owned)
-// global counters.  Counts are integers, so the result does not depend on the
-// order of the LDS atomics and equals the push result exactly.
+// in LDS, the events that the step's spikes of its incoming projections put on
+// each target of its tile, then writes the tile's counters once: no global
+// atomics.  Counters are 32 bits (a target receives at most one event per
+// spiking source of a projection); the padding slots of the runs count into
+// kPadSlots spare counters past the tile, which are never written out.  Counts
+// are integers, so the result does not depend on the order of the LDS atomics
+// and equals the push result exactly.  Projections are counted `pass` at a time
+// (as many as fit the LDS share that keeps kPullResident workgroups per compute
+// unit).
 //
-// Work split: each spike's run of targets in the tile (a contiguous slice of its
-// CSR row, about p * tile targets) is read by a group of 8 lanes with 16-byte
-// vector loads (8 tile-local 16-bit targets each), so one load instruction of a
-// group reads one 128-byte line; every group keeps kSpikesInFlight runs x
-// kRunUnroll vectors per lane in flight.  The run descriptors (first vector,
-// length, projection) of a batch of spikes are computed first, one spike per
-// thread, and staged in LDS.
+// Work split: a spike's run in the tile (a slice of its CSR row, about p * tile
+// targets, padded to whole 16-byte vectors of 8 tile-local targets) is read by
+// a group of kGroup lanes, lane l loading vectors l, l + kGroup, ...; a group
+// keeps kSpikesInFlight runs x kRunUnroll vectors per lane in flight.  Loads
+// are unconditional (a lane past the end of its run reloads the run's first
+// vector and does not count it), so they issue back to back and are consumed in
+// order; counting a vector is 8 LDS atomics with no per-target checks.  The run
+// descriptors of a batch of spikes (address of the first vector, vector count,
+// counter base) are computed first, one spike per thread, and staged in LDS.
 //
 // Workgroup L handles tile L / 16 and step L % 16: the steps of one block land on
 // fixed XCDs (workgroups are dealt to the 8 XCDs round-robin), so the spike
-// lists and row data that all workgroups of a step read are cached in one L2.
+// lists and run bounds that all workgroups of a step read are cached in one L2.
 struct PullArgs {
 	ProjDev const* projs;
 	PullTile const* tiles;
@@ -193,16 +196,29 @@ struct PullArgs {
 };
 
 constexpr int kPullThreads    = 512;
-constexpr int kGroup          = 8;    // lanes per spike run
-constexpr int kRunUnroll      = 4;    // vectors per lane per run in flight
-constexpr int kSpikesInFlight = 2;    // runs per group in flight
-constexpr int kDescBatch      = 1024; // spike descriptors staged in LDS at once
-constexpr int kPassProj       = 4;    // projections whose counters share LDS in one pass
+constexpr int kGroup          = 8;             // lanes per spike run
+constexpr int kRunUnroll      = 2;             // vectors per lane per run in flight
+constexpr int kSpikesInFlight = 2;             // runs per group in flight
+constexpr int kDescBatch      = kPullThreads;  // spike descriptors staged in LDS at once
+constexpr int kPassProj       = 4;             // most projections counted in one pass
+constexpr int kPadSlots       = 8;             // spare counters taking the padding slots
+constexpr int kPullResident   = 3;             // workgroups per compute unit the pass size aims for
 
 struct RunDesc {
-	uint32_t v0;    // first 16-byte vector of the run in btgt
-	uint32_t len;   // targets in the run
-	uint32_t meta;  // bits 0-2: offset of the run in its first vector; bits 3+: projection in pass
+	uint64_t addr;  // first 16-byte vector of the run
+	uint32_t nv;    // vectors in the run
+	uint32_t cnt;   // LDS word offset of the run's projection counters
+};
+
+// An incoming projection as seen from one (tile, step) workgroup.
+struct PassProj {
+	int32_t const* ids;      // the step's spiking sources
+	uint32_t const* rv;      // run bounds of the tile: rv[src * rvs], rv[src * rvs + 1]
+	uint16_t const* btgt;
+	int64_t const* offsets;  // CSR rows (event counts)
+	uint32_t* out;           // the tile's counters in the delivery slot
+	int64_t nsp;             // spikes
+	int64_t rvs;
 };
 
 constexpr size_t kLdsMax = 160 * 1024;  // LDS of a CDNA4 compute unit
@@ -225,127 +241,121 @@ __device__ inline void gstore(T* p, T v) {
 
 // LDS bytes of one pull workgroup counting `pass` projections of a tile together.
 size_t pull_lds_bytes(int64_t tile, int pass) {
-	return sizeof(uint32_t) * size_t(pass) * size_t((tile + 1) / 2) + sizeof(RunDesc) * kDescBatch;
+	return sizeof(uint32_t) * size_t(pass) * size_t(tile + kPadSlots) + sizeof(RunDesc) * kDescBatch +
+	       sizeof(PassProj) * SPICE_MAX_IN;
 }
 
-// Projections per pass for a tile: as many as the incoming projections need,
-// at most kPassProj, within the LDS of a compute unit (0: the tile is too large).
+// Projections per pass for a tile: as many as the incoming projections need
+// (at most kPassProj) while kPullResident workgroups fit a compute unit's LDS,
+// at least one (0: one projection's counters do not fit).
 int pull_pass(int64_t tile, int max_in) {
 	int p = std::min(kPassProj, std::max(max_in, 1));
-	while (p > 0 && pull_lds_bytes(tile, p) > kLdsMax) p--;
-	return p;
+	while (p > 1 && pull_lds_bytes(tile, p) > kLdsMax / kPullResident) p--;
+	return pull_lds_bytes(tile, p) <= kLdsMax ? p : 0;
 }
 
 __global__ void __launch_bounds__(kPullThreads) deliver_pull_kernel(PullArgs a) {
-	extern __shared__ uint32_t lds_cnt[];  // [kPassProj][words] two 16-bit counters per word, then descriptors
+	extern __shared__ uint32_t lds[];  // [pass][max_tile + kPadSlots] counters, then descriptors
 	int const k = int(blockIdx.x % SPICE_MAX_BLOCK);
 	if (k >= a.nsteps) return;
 	PullTile const pt    = a.tiles[blockIdx.x / SPICE_MAX_BLOCK];
 	PullTarget const& tg = a.targets[pt.target];
 	int64_t const step   = local_step(a.win, tg.inst, a.rel + k);
 	if (step < 0) return;
-	int64_t const T0    = int64_t(pt.tau) * tg.tile;
-	int64_t const Tn    = min<int64_t>(tg.tile, tg.dst_size - T0);
-	int64_t const words = (tg.tile + 1) / 2;
-	RunDesc* desc       = reinterpret_cast<RunDesc*>(lds_cnt + a.pass * ((a.max_tile + 1) / 2));
+	int64_t const T0     = int64_t(pt.tau) * tg.tile;
+	int const Tn         = int(min<int64_t>(tg.tile, tg.dst_size - T0));
+	int const stride     = int(tg.tile) + kPadSlots;  // counters per projection
+	RunDesc* desc        = reinterpret_cast<RunDesc*>(lds + a.pass * (a.max_tile + kPadSlots));
+	PassProj* pp         = reinterpret_cast<PassProj*>(desc + kDescBatch);
 	int const tid = threadIdx.x, lane = tid % kWave, group = tid / kGroup, gl = tid % kGroup;
 	constexpr int kGroups = kPullThreads / kGroup;
 	int64_t const slot_in = step % a.ring;
 	unsigned long long ev = 0, rows = 0;
-
-	for (int j0 = 0; j0 < tg.nin; j0 += a.pass) {
-		int const np = min(a.pass, tg.nin - j0);
-		int64_t nsp[kPassProj];
-		int64_t most = 0;
-		for (int j = 0; j < kPassProj; j++) {
-			nsp[j] = j < np ? gload(a.projs[tg.proj[j0 + j]].src_ring_cnt + slot_in) : 0;
-			most   = max(most, nsp[j]);
-			if (pt.tau == 0 && tid == 0) rows += uint64_t(nsp[j]);
-		}
-		for (int64_t c0 = 0; c0 == 0 || c0 < most; c0 += 65535) {
-			// spikes of this chunk, projection by projection
-			int64_t pre[kPassProj + 1];
-			pre[0] = 0;
-			for (int j = 0; j < kPassProj; j++)
-				pre[j + 1] = pre[j] + max<int64_t>(0, min<int64_t>(nsp[j], c0 + 65535) - c0);
-			int64_t const M = pre[kPassProj];
-			for (int64_t i = tid; i < int64_t(np) * words; i += kPullThreads) lds_cnt[i] = 0;
-			for (int64_t d0 = 0; d0 < M; d0 += kDescBatch) {
-				int64_t const nd = min<int64_t>(kDescBatch, M - d0);
-				__syncthreads();  // counters cleared / previous descriptors consumed
-				for (int64_t f = tid; f < nd; f += kPullThreads) {
-					int64_t const g = d0 + f;
-					int j           = 0;
-					while (j + 1 < kPassProj && g >= pre[j + 1]) j++;
-					ProjDev const& P   = a.projs[tg.proj[j0 + j]];
-					int64_t const src  = gload(P.src_ring_ids + slot_in * P.src_size + c0 + (g - pre[j]));
-					int64_t const base = gload(P.offsets + src);
-					uint16_t const* rt = P.rt + src * (P.ntile + 1) + pt.tau;
-					int64_t const lo   = base + gload(rt);
-					int64_t const hi   = base + gload(rt + 1);
-					desc[f]            = {uint32_t(lo >> 3), uint32_t(hi - lo), uint32_t(lo & 7) | uint32_t(j << 3)};
-					ev += uint64_t(hi - lo);
-				}
-				__syncthreads();
-				for (int64_t r0 = group; r0 < nd; r0 += int64_t(kGroups) * kSpikesInFlight) {
-					RunDesc d[kSpikesInFlight];
-					uint16_t const* bt[kSpikesInFlight];
-					uint32_t* cnt[kSpikesInFlight];
-					uint32_t nv[kSpikesInFlight], most_nv = 0;
-#pragma unroll
-					for (int r = 0; r < kSpikesInFlight; r++) {
-						int64_t const idx = r0 + int64_t(r) * kGroups;
-						d[r]              = idx < nd ? desc[idx] : RunDesc{0, 0, 0};
-						int const j       = int(d[r].meta >> 3);
-						bt[r]             = a.projs[tg.proj[j0 + j]].btgt;
-						cnt[r]            = lds_cnt + j * words;
-						nv[r]             = d[r].len ? ((d[r].meta & 7) + d[r].len + 7) >> 3 : 0;
-						most_nv           = max(most_nv, nv[r]);
-					}
-					for (uint32_t vb = 0; vb < most_nv; vb += kGroup * kRunUnroll) {
-						u32x4 w[kSpikesInFlight][kRunUnroll];
-#pragma unroll
-						for (int r = 0; r < kSpikesInFlight; r++)
-#pragma unroll
-							for (int u = 0; u < kRunUnroll; u++) {
-								uint32_t const vi = vb + uint32_t(u * kGroup + gl);
-								if (vi < nv[r])
-									w[r][u] = gload(reinterpret_cast<u32x4 const*>(bt[r] + (uint64_t(d[r].v0) + vi) * 8));
-							}
-#pragma unroll
-						for (int r = 0; r < kSpikesInFlight; r++)
-#pragma unroll
-							for (int u = 0; u < kRunUnroll; u++) {
-								uint32_t const vi = vb + uint32_t(u * kGroup + gl);
-								if (vi >= nv[r]) continue;
-								uint32_t const wd[4] = {w[r][u].x, w[r][u].y, w[r][u].z, w[r][u].w};
-								int32_t const first  = int32_t(vi * 8) - int32_t(d[r].meta & 7);
-#pragma unroll
-								for (int e = 0; e < 8; e++) {
-									int32_t const pos = first + e;  // position within the run
-									if (pos >= 0 && pos < int32_t(d[r].len)) {
-										uint32_t const t = (wd[e >> 1] >> (16 * (e & 1))) & 0xffffu;
-										atomicAdd(&cnt[r][t >> 1], 1u << (16 * (t & 1)));
-									}
-								}
-							}
-					}
-				}
-			}
-			__syncthreads();
-			for (int j = 0; j < np; j++) {
-				ProjDev const& P = a.projs[tg.proj[j0 + j]];
-				uint32_t* out    = P.counts + ((step + P.delay) % P.slots) * P.dst_size + T0;
-				uint32_t const* cj = lds_cnt + j * words;
-				for (int64_t i = tid; i < Tn; i += kPullThreads) {
-					uint32_t const c = (cj[i >> 1] >> (16 * (i & 1))) & 0xffffu;
-					gstore(out + i, c0 == 0 ? c : gload(out + i) + c);
-				}
-			}
-			__syncthreads();
-		}
+	int const nin = tg.nin;
+	if (tid < nin) {  // the incoming projections as seen from this tile and step
+		ProjDev const& P = a.projs[tg.proj[tid]];
+		pp[tid] = {P.src_ring_ids + slot_in * P.src_size, P.rv + pt.tau, P.btgt, P.offsets,
+		           P.counts + ((step + P.delay) % P.slots) * P.dst_size + T0, gload(P.src_ring_cnt + slot_in),
+		           P.ntile + 1};
 	}
-	(void)lane;
+	__syncthreads();
+
+	for (int j0 = 0; j0 < nin; j0 += a.pass) {
+		int const np = min(a.pass, nin - j0);
+		int64_t pre[kPassProj + 1];  // spikes of the pass's projections, prefix
+		pre[0] = 0;
+#pragma unroll
+		for (int j = 0; j < kPassProj; j++) pre[j + 1] = pre[j] + (j < np ? pp[j0 + j].nsp : 0);
+		int64_t const M = pre[kPassProj];
+		if (pt.tau == 0 && tid == 0) rows += uint64_t(M);
+		for (int i = tid; i < np * stride; i += kPullThreads) lds[i] = 0;
+		for (int64_t d0 = 0; d0 < M; d0 += kDescBatch) {
+			int const nd = int(min<int64_t>(kDescBatch, M - d0));
+			__syncthreads();  // counters cleared / previous descriptors consumed
+			if (tid < nd) {
+				int64_t const g = d0 + tid;
+				int j           = 0;
+#pragma unroll
+				for (int q = 1; q < kPassProj; q++) j += g >= pre[q] ? 1 : 0;
+				PassProj const& q = pp[j0 + j];
+				int64_t const src = gload(q.ids + (g - pre[j]));
+				uint32_t const* r = q.rv + src * q.rvs;
+				uint32_t const lo = gload(r), hi = gload(r + 1);
+				desc[tid] = {reinterpret_cast<uint64_t>(q.btgt + uint64_t(lo) * 8), hi - lo, uint32_t(j * stride)};
+				if (pt.tau == 0) ev += uint64_t(gload(q.offsets + src + 1) - gload(q.offsets + src));
+			}
+			__syncthreads();
+			for (int r0 = group; r0 < nd; r0 += kGroups * kSpikesInFlight) {
+				u32x4 const* v[kSpikesInFlight];
+				uint32_t nv[kSpikesInFlight], cb[kSpikesInFlight], most = 0;
+#pragma unroll
+				for (int r = 0; r < kSpikesInFlight; r++) {
+					int const idx   = r0 + r * kGroups;
+					RunDesc const d = desc[idx < nd ? idx : r0];
+					v[r]            = reinterpret_cast<u32x4 const*>(d.addr);
+					nv[r]           = idx < nd ? d.nv : 0;
+					cb[r]           = d.cnt;
+					most            = max(most, nv[r]);
+				}
+				for (uint32_t vb = 0; vb < most; vb += kGroup * kRunUnroll) {
+					u32x4 w[kSpikesInFlight][kRunUnroll];
+#pragma unroll
+					for (int r = 0; r < kSpikesInFlight; r++)
+#pragma unroll
+						for (int u = 0; u < kRunUnroll; u++) {
+							uint32_t const vi = vb + uint32_t(u * kGroup + gl);
+							w[r][u]           = gload(v[r] + (vi < nv[r] ? vi : 0));
+						}
+					// keep every load issued here, ahead of the counting (the compiler
+					// would otherwise move a load into the branch that consumes it)
+					asm volatile("" ::: "memory");
+#pragma unroll
+					for (int r = 0; r < kSpikesInFlight; r++)
+#pragma unroll
+						for (int u = 0; u < kRunUnroll; u++) {
+							uint32_t const vi = vb + uint32_t(u * kGroup + gl);
+							if (vi < nv[r]) {
+								uint32_t* c = lds + cb[r];
+								atomicAdd(c + (w[r][u].x & 0xffffu), 1u);
+								atomicAdd(c + (w[r][u].x >> 16), 1u);
+								atomicAdd(c + (w[r][u].y & 0xffffu), 1u);
+								atomicAdd(c + (w[r][u].y >> 16), 1u);
+								atomicAdd(c + (w[r][u].z & 0xffffu), 1u);
+								atomicAdd(c + (w[r][u].z >> 16), 1u);
+								atomicAdd(c + (w[r][u].w & 0xffffu), 1u);
+								atomicAdd(c + (w[r][u].w >> 16), 1u);
+							}
+						}
+				}
+			}
+		}
+		__syncthreads();
+		for (int j = 0; j < np; j++) {
+			uint32_t* out = pp[j0 + j].out;
+			for (int i = tid; i < Tn; i += kPullThreads) gstore(out + i, lds[j * stride + i]);
+		}
+		__syncthreads();  // counters written before the next pass clears them
+	}
 	for (int off = 32; off > 0; off >>= 1) ev += __shfl_xor(ev, off, kWave);
 	if (lane == 0 && ev) atomicAdd(a.events, ev);
 	if (tid == 0 && rows) atomicAdd(a.events + 1, rows);
@@ -465,7 +475,7 @@ int refresh_device_descriptors(spice_ctx* ctx) {
 	for (auto& Q : ctx->projs) {
 		Pop const& S = ctx->pops[Q.src];
 		qd.push_back({Q.dead ? -1 : int64_t(S.inst), Q.offsets, Q.neighbors, Q.counts, Q.slots, ctx->pops[Q.dst].size, Q.delay,
-		              S.ring_ids, S.ring_cnt, S.size, Q.rt, Q.btgt, Q.ntile, Q.ev_offsets});
+		              S.ring_ids, S.ring_cnt, S.size, Q.rv, Q.btgt, Q.ntile, Q.ev_offsets});
 	}
 	// projection views for the synapse kernels; ids of stateful projections
 	std::vector<spice_proj_view> pvs;
@@ -724,7 +734,7 @@ void spice_ctx_destroy(spice_ctx* ctx) {
 		(void)hipFree(Q.offsets);
 		(void)hipFree(Q.neighbors);
 		(void)hipFree(Q.counts);
-		(void)hipFree(Q.rt);
+		(void)hipFree(Q.rv);
 		(void)hipFree(Q.btgt);
 		(void)hipFree(Q.syn_state);
 		(void)hipFree(Q.ev_offsets);
@@ -767,7 +777,8 @@ int spice_ctx_set_delivery(spice_ctx* ctx, int mode) {
 }
 
 int spice_ctx_set_tile(spice_ctx* ctx, int64_t tile) {
-	if (tile < 64 || tile > 65536 || tile % 64) return set_error(SPICE_EINVAL, "tile must be a multiple of 64 in [64, 65536]");
+	if (tile < 64 || tile > kMaxPullTile || tile % 64)
+		return set_error(SPICE_EINVAL, "tile must be a multiple of 64 in [64, " + std::to_string(kMaxPullTile) + "]");
 	ctx->tile      = tile;
 	ctx->dev_dirty = true;
 	return SPICE_OK;
@@ -777,8 +788,8 @@ int64_t spice_ctx_tile(spice_ctx* ctx) { return ctx->tile; }
 
 int spice_pop_set_tile(spice_ctx* ctx, int64_t pop, int64_t tile) {
 	if (pop < 0 || pop >= int64_t(ctx->pops.size())) return set_error(SPICE_EINVAL, "bad population");
-	if (tile != 0 && (tile < 64 || tile > 65536 || tile % 64))
-		return set_error(SPICE_EINVAL, "tile must be 0 or a multiple of 64 in [64, 65536]");
+	if (tile != 0 && (tile < 64 || tile > kMaxPullTile || tile % 64))
+		return set_error(SPICE_EINVAL, "tile must be 0 or a multiple of 64 in [64, " + std::to_string(kMaxPullTile) + "]");
 	if (ctx->pops[pop].tile != tile) {
 		ctx->pops[pop].tile = tile;
 		ctx->dev_dirty      = true;
@@ -854,7 +865,7 @@ int spice_instance_destroy(spice_ctx* ctx, int64_t inst) {
 	for (auto& Q : ctx->projs) {
 		if (Q.dead || ctx->pops[Q.src].inst != inst) continue;
 		for (void* p : {static_cast<void*>(Q.offsets), static_cast<void*>(Q.neighbors), static_cast<void*>(Q.counts),
-		                static_cast<void*>(Q.rt), static_cast<void*>(Q.btgt), static_cast<void*>(Q.syn_state),
+		                static_cast<void*>(Q.rv), static_cast<void*>(Q.btgt), static_cast<void*>(Q.syn_state),
 		                static_cast<void*>(Q.ev_offsets), static_cast<void*>(Q.ev_cursor),
 		                static_cast<void*>(Q.events), static_cast<void*>(Q.ages)})
 			(void)hipFree(p);
@@ -1203,8 +1214,9 @@ int spice_ctx_autotune(spice_ctx* ctx, int64_t const* candidates, int32_t ncand,
 	if (ctx->last_rel < 0) return set_error(SPICE_EINVAL, "tile tuning replays the last block: run steps first");
 	if (ncand < 1 || reps < 1) return set_error(SPICE_EINVAL, "no tile candidates");
 	for (int32_t c = 0; c < ncand; c++)
-		if (candidates[c] < 64 || candidates[c] > 65536 || candidates[c] % 64)
-			return set_error(SPICE_EINVAL, "tile candidates must be multiples of 64 in [64, 65536]");
+		if (candidates[c] < 64 || candidates[c] > kMaxPullTile || candidates[c] % 64)
+			return set_error(SPICE_EINVAL, "tile candidates must be multiples of 64 in [64, " +
+			                                   std::to_string(kMaxPullTile) + "]");
 	SPICE_HIP(hipSetDevice(ctx->device));
 	// pull delivery overwrites the counts of the block it delivers, so replaying
 	// the last block is idempotent; only the event counters are restored

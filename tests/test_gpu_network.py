@@ -141,7 +141,7 @@ def test_sample_program_output_is_byte_identical():
     assert hashlib.md5(r.stdout).hexdigest() == "dc05acddc0e3fb6da4e490641121fbe9"
 
 
-@pytest.mark.parametrize("tile", [64, 640, 4096, 65536])
+@pytest.mark.parametrize("tile", [64, 640, 4096, 32768])
 def test_pull_tile_sizes(tile):
     """Pull delivery gives the reference raster for any tile size (tiles that do
     not divide the population, a single tile, tiles of 64 targets)."""
