@@ -151,11 +151,27 @@ __device__ inline void apply_events(N& n, Syn const& syn, uint32_t const* ev, ui
 	}
 }
 
+// deliver() k times, exactly as k calls in a row: in blocks of 8 calls, each on
+// a copy that is kept only for the first k (a select, not a branch), so a wave
+// loops over blocks of its lanes' largest count instead of over single events.
+// A stateless synapse's deliver() changes nothing but the neuron passed to it.
+template <class Syn, class N>
+__device__ inline void deliver_times(N& n, Syn const& syn, uint32_t k) {
+	for (uint32_t r = 0; r < k; r += 8) {
+#pragma unroll
+		for (uint32_t j = 0; j < 8; j++) {
+			N t = n;
+			syn.deliver(t);
+			if (r + j < k) n = t;
+		}
+	}
+}
+
 template <class Syn, class N>
 __device__ void replay_thunk(void const* syn, void* neuron, uint32_t k) {
 	Syn const& s = *static_cast<Syn const*>(syn);
 	N& n         = *static_cast<N*>(neuron);
-	for (uint32_t i = 0; i < k; i++) s.deliver(n);
+	deliver_times(n, s, k);
 }
 
 template <class Syn, class N>
@@ -217,6 +233,11 @@ __device__ inline spice_step step_of(spice_block const& b, int32_t inst, int u);
 // Event counts of neuron i for application step `local`, all incoming
 // projections at once (independent loads, issued together).  MaxIn bounds the
 // population's incoming projections (a compile-time size keeps k in registers).
+__device__ inline int64_t ring_slot(int64_t local, int64_t slots) {
+	if ((uint64_t(local) | uint64_t(slots)) < (uint64_t(1) << 32)) return int64_t(uint32_t(local) % uint32_t(slots));
+	return local % slots;
+}
+
 template <int MaxIn>
 __device__ inline void load_counts(uint32_t (&k)[MaxIn], spice_pop_view const& v, int64_t local, int64_t i,
                                    bool clear) {
@@ -224,7 +245,7 @@ __device__ inline void load_counts(uint32_t (&k)[MaxIn], spice_pop_view const& v
 	for (int c = 0; c < MaxIn; c++) {
 		k[c] = 0;
 		if (c < v.nin) {
-			uint32_t* cp = v.in[c].counts + (local % v.in[c].slots) * v.size + i;
+			uint32_t* cp = v.in[c].counts + ring_slot(local, v.in[c].slots) * v.size + i;
 			k[c]         = gload32(cp);
 			if (clear && k[c]) gstore32(cp, 0);
 		}
@@ -249,6 +270,14 @@ __device__ inline T cload_obj(void const* p) {
 __device__ inline spice_step step_of(spice_block const& b, int32_t inst, int u) {
 	spice_step const* steps = cload_obj<spice_window>(b.win).steps;
 	return cload_obj<spice_step>(steps + int64_t(inst) * SPICE_MAX_WINDOW + b.rel + u);
+}
+
+// The block's steps of one instance, loaded once by the first threads of the
+// workgroup into LDS (a kernel that walks the steps then reads each from LDS
+// instead of waiting on a table load per step).  Every thread must call it.
+__device__ inline void stage_steps(spice_block const& b, int32_t inst, spice_step* sts) {
+	if (int(threadIdx.x) < b.nsteps) sts[threadIdx.x] = step_of(b, inst, int(threadIdx.x));
+	__syncthreads();
 }
 
 // The incoming projections of a population, loaded once per kernel: replay
@@ -291,7 +320,7 @@ __device__ inline void replay(N& n, uint32_t const (&k)[MaxIn], spice_pop_view c
 						spice_in_view const& in = v.in[c];
 						apply_events<Syn>(n, syn, in.events + in.ev_offsets[i], k[c], in.syn_state, in.nnz);
 					} else {
-						for (uint32_t r = 0; r < k[c]; r++) syn.deliver(n);
+						deliver_times(n, syn, k[c]);
 					}
 					done = true;
 				}
@@ -321,8 +350,94 @@ __device__ inline bool call_update(Neur const& neur, typename neuron_traits<Neur
 		return neur.update(dt, rng);
 }
 
-// One thread per neuron (populations whose neurons take no draws).  The counts
-// of step u+1 are loaded while step u is computed.
+// One thread per neuron (populations whose neurons take no draws).  Every count
+// a block's steps read was delivered before the block began (blocks are no
+// longer than the shortest delay), so the counts of step u + A are loaded while
+// step u is computed, A = counts_ahead steps ahead.  Every wave issues the same
+// memory operations in the same order whatever its lanes hold (loads past the
+// block or past nin read valid addresses and are ignored; waves past the end of
+// the population store their spike word to the row's scratch word), so that a
+// step waits only for its own counts, not for the loads issued after them.
+template <int MaxIn>
+constexpr int counts_ahead = MaxIn <= 4 ? 4 : 2;
+
+template <class Neur, class Syn, int MaxIn, bool Mixed, bool Clear, class N>
+__device__ inline void update_lanes_steps(spice_block const& b, spice_pop_view const& v, Neur const& neur,
+                                          synapse_set<Syn, MaxIn> const& ss, spice_step const* sts, N& n,
+                                          int64_t i, bool active) {
+	constexpr int A   = counts_ahead<MaxIn>;
+	constexpr bool Nv = std::is_void_v<typename neuron_traits<Neur>::type>;
+	int const lane    = threadIdx.x % 64;
+	// The block's steps are consecutive: the count slot of step u of projection
+	// c is slot0[c] + u, wrapped once (rings are longer than a block plus A).
+	uint32_t* crow[MaxIn];
+	int64_t slot0[MaxIn], nslot[MaxIn];
+	int64_t const local0 = sts[0].local;
+#pragma unroll
+	for (int c = 0; c < MaxIn; c++) {
+		int const cc = c < v.nin ? c : 0;
+		if (v.nin > 0) {
+			crow[c]  = v.in[cc].counts + (active ? i : 0);
+			nslot[c] = v.in[cc].slots;
+			slot0[c] = local0 >= 0 ? ring_slot(local0, nslot[c]) : 0;
+		} else {  // nothing to count: any valid address
+			crow[c]  = v.state;
+			nslot[c] = 1;
+			slot0[c] = 0;
+		}
+	}
+	auto counts_of = [&](int u, int c) {
+		int64_t s = slot0[c] + u;
+		if (s >= nslot[c]) s -= nslot[c];
+		return crow[c] + (v.nin > 0 ? s * v.size : 0);
+	};
+	uint32_t kb[A][MaxIn] = {};  // counts of steps u0 + q, q < A
+	if constexpr (!Nv) {
+#pragma unroll
+		for (int q = 0; q < A; q++)
+#pragma unroll
+			for (int c = 0; c < MaxIn; c++) kb[q][c] = gload32(counts_of(q, c));
+	}
+	int64_t const word = (i >> 6) < v.bitmap_stride - 1 ? (i >> 6) : v.bitmap_stride - 1;
+	bool done = false;  // an instance's steps in a window form a prefix
+	for (int u0 = 0; u0 < b.nsteps && !done; u0 += A) {
+#pragma unroll
+		for (int q = 0; q < A; q++) {
+			int const u = u0 + q;
+			if (u >= b.nsteps || done) continue;
+			spice_step const st = sts[u];
+			if (st.local < 0) {
+				done = true;
+				continue;
+			}
+			uint32_t k[MaxIn];
+#pragma unroll
+			for (int c = 0; c < MaxIn; c++) k[c] = c < v.nin && active ? kb[q][c] : 0;
+			if constexpr (!Nv) {
+#pragma unroll
+				for (int c = 0; c < MaxIn; c++) kb[q][c] = gload32(counts_of(u + A, c));
+				if constexpr (Clear)  // push delivery accumulates: consumed counts restart at zero
+#pragma unroll
+					for (int c = 0; c < MaxIn; c++)
+						if (c < v.nin && active) gstore32(counts_of(u, c), 0u);
+			}
+			bool spiked = false;
+			if (active) {
+				if constexpr (!Nv) replay<Syn, MaxIn, Mixed>(n, k, v, i, ss);
+				device_stream rng{st.seed.s0, st.seed.s1, 0};
+				if constexpr (Nv)
+					spiked = call_update<Neur>(neur, nullptr, st.dt, rng);
+				else
+					spiked = call_update<Neur>(neur, &n, st.dt, rng);
+				if (rng.count != 0) atomicExch(v.error, 1);
+			}
+			uint64_t const bits = __ballot(spiked);
+			(void)lane;
+			gstore64(v.bitmap + u * v.bitmap_stride + word, bits);
+		}
+	}
+}
+
 template <class Neur, class Syn, int MaxIn, bool Mixed>
 __global__ void __launch_bounds__(256) update_lanes_kernel(spice_block b, group_args<Neur> g) {
 	using N                 = typename neuron_traits<Neur>::type;
@@ -333,42 +448,15 @@ __global__ void __launch_bounds__(256) update_lanes_kernel(spice_block b, group_
 	if constexpr (!std::is_void_v<typename neuron_traits<Neur>::type>) ss.load(g.slots[e.member], v.nin);
 	int64_t const i         = e.first + threadIdx.x;
 	bool const active       = i < v.size;
-	int const lane          = threadIdx.x % 64;
-	bool const clear        = v.clear_counts != 0;
 	std::conditional_t<std::is_void_v<N>, char, N> n{};
 	if constexpr (!std::is_void_v<N>)
 		if (active) load_state(n, v.state, v.size, i);
-	uint32_t k[MaxIn] = {}, kn[MaxIn] = {};
-	if constexpr (!std::is_void_v<N>) {
-		spice_step const s0 = step_of(b, v.inst, 0);
-		if (active && s0.local >= 0) load_counts<MaxIn>(k, v, s0.local, i, clear);
-	}
-	for (int u = 0; u < b.nsteps; u++) {
-		spice_step const st = step_of(b, v.inst, u);
-		if (st.local < 0) break;  // an instance's steps in a window form a prefix
-		bool spiked = false;
-		if constexpr (!std::is_void_v<N>) {
-			if (u + 1 < b.nsteps) {
-				spice_step const sn = step_of(b, v.inst, u + 1);
-				if (active && sn.local >= 0) load_counts<MaxIn>(kn, v, sn.local, i, clear);
-			}
-		}
-		if (active) {
-			if constexpr (!std::is_void_v<N>) replay<Syn, MaxIn, Mixed>(n, k, v, i, ss);
-			device_stream rng{st.seed.s0, st.seed.s1, 0};
-			if constexpr (std::is_void_v<N>)
-				spiked = call_update<Neur>(neur, nullptr, st.dt, rng);
-			else
-				spiked = call_update<Neur>(neur, &n, st.dt, rng);
-			if (rng.count != 0) atomicExch(v.error, 1);
-		}
-		uint64_t const bits = __ballot(spiked);
-		if (lane == 0 && (i >> 6) < v.bitmap_stride) gstore64(v.bitmap + u * v.bitmap_stride + (i >> 6), bits);
-		if constexpr (!std::is_void_v<N>) {
-#pragma unroll
-			for (int c = 0; c < MaxIn; c++) k[c] = kn[c];
-		}
-	}
+	__shared__ spice_step sts[SPICE_MAX_BLOCK];
+	stage_steps(b, v.inst, sts);
+	if (v.clear_counts != 0)
+		update_lanes_steps<Neur, Syn, MaxIn, Mixed, true>(b, v, neur, ss, sts, n, i, active);
+	else
+		update_lanes_steps<Neur, Syn, MaxIn, Mixed, false>(b, v, neur, ss, sts, n, i, active);
 	if constexpr (!std::is_void_v<N>)
 		if (active) store_state(n, v.state, v.size, i);
 }
@@ -393,9 +481,18 @@ __global__ void __launch_bounds__(256) update_stateless_kernel(spice_block b, gr
 	jump_stream(rng, cload_obj<spice_rng_state>(v.chunk_jump + c * 4));
 	int const n   = int(v.size - i0 < 64 ? v.size - i0 : 64);
 	uint64_t bits = 0;
-	for (int t = 0; t < n; t++) {
-		bool const spiked = call_update<Neur>(neur, nullptr, st.dt, rng);
-		bits |= uint64_t(spiked) << t;
+	if (n == 64) {  // whole chunk: constant shifts, one 32-bit word at a time
+		uint32_t w[2] = {0, 0};
+#pragma unroll
+		for (int h = 0; h < 2; h++)
+#pragma unroll
+			for (int t = 0; t < 32; t++) w[h] |= uint32_t(call_update<Neur>(neur, nullptr, st.dt, rng)) << t;
+		bits = uint64_t(w[0]) | (uint64_t(w[1]) << 32);
+	} else {
+		for (int t = 0; t < n; t++) {
+			bool const spiked = call_update<Neur>(neur, nullptr, st.dt, rng);
+			bits |= uint64_t(spiked) << t;
+		}
 	}
 	if (rng.count != n * v.draws) atomicExch(v.error, 1);
 	gstore64(v.bitmap + u * v.bitmap_stride + c, bits);
@@ -419,8 +516,10 @@ __global__ void __launch_bounds__(256) update_chunks_kernel(spice_block b, group
 	int const lane          = threadIdx.x % 64;
 	bool const clear        = v.clear_counts != 0;
 	spice_rng_state const jp = active ? v.chunk_jump[chunk] : spice_rng_state{1, 0};
+	__shared__ spice_step sts[SPICE_MAX_BLOCK];
+	stage_steps(b, v.inst, sts);
 	for (int u = 0; u < b.nsteps; u++) {
-		spice_step const st = step_of(b, v.inst, u);
+		spice_step const st = sts[u];
 		if (st.local < 0) break;
 		uint32_t bits = 0;
 		if (active) {

@@ -221,7 +221,12 @@ SPICE_HOST_DEVICE constexpr Real generate_canonical(auto& rng) {
 
 	UInt bits = rng();
 	if constexpr (rng_size < sizeof(Real)) bits = (bits << (8 * rng_size)) | rng();
-	return Real((bits >> (width - digits)) + (LeftOpen ? 1u : 0u)) / Real(1_u64 << digits);
+	UInt const v = (bits >> (width - digits)) + (LeftOpen ? 1u : 0u);
+	// v <= 2^digits: below 2^32 it converts from 32 bits (exactly, as from 64)
+	if constexpr (digits < 32)
+		return Real(uint32_t(v)) / Real(1_u64 << digits);
+	else
+		return Real(v) / Real(1_u64 << digits);
 }
 
 template <std::floating_point Real, bool LeftOpen = false>

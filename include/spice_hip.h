@@ -67,7 +67,8 @@ typedef struct {
 	int64_t size;
 	int32_t words;          /* 32-bit words per neuron (sizeof(neuron) rounded up) */
 	int32_t draws;          /* RNG outputs each neuron takes per step (stream offset stride) */
-	uint64_t* bitmap;       /* [SPICE_MAX_BLOCK][bitmap_stride] spike bits of the running block */
+	uint64_t* bitmap;       /* [SPICE_MAX_BLOCK][bitmap_stride] spike bits of the running block;
+	                           the last word of a row is scratch (update kernels' waves past the end) */
 	int64_t bitmap_stride;
 	/* per chunk of 16 neurons: x^(stream offset of the chunk's first draw) mod P,
 	 * P the generator's characteristic polynomial (offset counted from the

@@ -67,7 +67,10 @@ __device__ inline int64_t local_step(spice_window const* win, int64_t inst, int 
 }
 
 // Spike bits of one (population, step) -> ascending id list in the ring slot.
-__global__ void __launch_bounds__(256)
+// Each thread owns a run of consecutive bitmap words; a workgroup scan of the
+// threads' spike counts places every thread's ids.
+constexpr int kCompactThreads = 1024;
+__global__ void __launch_bounds__(kCompactThreads)
 compact_kernel(PopDev const* pops, spice_window const* win, int rel, int64_t ring) {
 	PopDev const P  = pops[blockIdx.x];
 	int const k     = blockIdx.y;
@@ -79,23 +82,32 @@ compact_kernel(PopDev const* pops, spice_window const* win, int rel, int64_t rin
 	int64_t const slot = t % ring;
 	int32_t* out       = P.ring_ids + slot * P.size;
 
-	int const tid      = threadIdx.x;
-	int64_t const per  = (nw + blockDim.x - 1) / blockDim.x;
+	int const tid      = threadIdx.x, lane = tid % 64, wave = tid / 64;
+	constexpr int kWaves = kCompactThreads / 64;
+	int64_t const per  = (nw + kCompactThreads - 1) / kCompactThreads;
 	int64_t const w0   = min<int64_t>(nw, tid * per);
 	int64_t const w1   = min<int64_t>(nw, w0 + per);
 	int cnt            = 0;
 	for (int64_t w = w0; w < w1; w++) cnt += __popcll(bm[w]);
 
-	__shared__ int scan[256];
-	scan[tid] = cnt;
-	__syncthreads();
-	for (int off = 1; off < 256; off <<= 1) {
-		int const v = tid >= off ? scan[tid - off] : 0;
-		__syncthreads();
-		scan[tid] += v;
-		__syncthreads();
+	int x = cnt;  // inclusive scan within the wave, then over the wave totals
+	for (int off = 1; off < 64; off <<= 1) {
+		int const y = __shfl_up(x, off, 64);
+		if (lane >= off) x += y;
 	}
-	int pos = scan[tid] - cnt;
+	__shared__ int wsum[kWaves];
+	if (lane == 63) wsum[wave] = x;
+	__syncthreads();
+	if (wave == 0) {
+		int y = lane < kWaves ? wsum[lane] : 0;
+		for (int off = 1; off < kWaves; off <<= 1) {
+			int const z = __shfl_up(y, off, 64);
+			if (lane >= off) y += z;
+		}
+		if (lane < kWaves) wsum[lane] = y;
+	}
+	__syncthreads();
+	int pos = (wave > 0 ? wsum[wave - 1] : 0) + x - cnt;
 	for (int64_t w = w0; w < w1; w++) {
 		uint64_t m = bm[w];
 		while (m) {
@@ -104,7 +116,7 @@ compact_kernel(PopDev const* pops, spice_window const* win, int rel, int64_t rin
 			m &= m - 1;
 		}
 	}
-	if (tid == 255) P.ring_cnt[slot] = scan[255];
+	if (tid == 0) P.ring_cnt[slot] = wsum[kWaves - 1];
 }
 
 // Spike history of populations with plastic projections (neuron_population.h:122-128):
@@ -652,7 +664,7 @@ int run_blocks(spice_ctx* ctx, int32_t nsteps, int32_t L, bool timed) {
 		}
 		if (timed) SPICE_HIP(hipEventRecord(tb.ev[1], ctx->stream));
 		if (!ctx->pops.empty()) {
-			hipLaunchKernelGGL(compact_kernel, dim3(unsigned(ctx->pops.size()), unsigned(n)), dim3(256), 0,
+			hipLaunchKernelGGL(compact_kernel, dim3(unsigned(ctx->pops.size()), unsigned(n)), dim3(kCompactThreads), 0,
 			                   ctx->stream, ctx->pops_dev, ctx->win_dev, rel, ctx->ring);
 			SPICE_HIP(hipGetLastError());
 		}
@@ -911,7 +923,7 @@ int spice_pop_create_in(spice_ctx* ctx, int64_t inst, int64_t size, int32_t stat
 	P.size          = size;
 	P.words         = state_words;
 	P.draws         = draws;
-	P.bitmap_stride = (size + 63) / 64;
+	P.bitmap_stride = (size + 63) / 64 + 1;  // one spare word per row: scratch for waves past the end
 	if (int r = dev_alloc(&P.state, size_t(size) * size_t(std::max(state_words, 0)))) return r;
 	if (int r = dev_alloc(&P.bitmap, size_t(SPICE_MAX_BLOCK) * P.bitmap_stride)) return r;
 	if (int r = dev_alloc(&P.ring_ids, size_t(ctx->ring) * size_t(size), false)) return r;
