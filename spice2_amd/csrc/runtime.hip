@@ -209,7 +209,7 @@ struct PullArgs {
 
 constexpr int kPullThreads    = 512;
 constexpr int kGroup          = 8;             // lanes per spike run
-constexpr int kRunUnroll      = 2;             // vectors per lane per run in flight
+constexpr int kRunUnroll      = 4;             // vectors per lane per run in flight
 constexpr int kSpikesInFlight = 2;             // runs per group in flight
 constexpr int kDescBatch      = kPullThreads;  // spike descriptors staged in LDS at once
 constexpr int kPassProj       = 4;             // most projections counted in one pass
