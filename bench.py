@@ -246,7 +246,7 @@ def serve_requests(n: int, rate: float, seed: int = 2025):
     return kind, N, p, steps, arrival, [(2025, i) for i in range(n)]
 
 
-def run_serve(n: int, rate: float, budget_gb: float, window: int, alone_sample: int):
+def run_serve(n: int, rate: float, budget_gb: float, window: int, alone_sample: int, graphs: bool = True):
     """The job stream through spice::server; p50/p99 latency (arrival to
     finish), aggregate synaptic events/s over the run, and a sample of jobs
     re-run alone whose results must be identical."""
@@ -254,7 +254,8 @@ def run_serve(n: int, rate: float, budget_gb: float, window: int, alone_sample: 
     kind, N, p, steps, arr, seeds = serve_requests(n, rate)
     fp = [S.job_footprint(int(k), int(x), float(q)) for k, x, q in zip(kind, N, p)]
     t0 = time.perf_counter()
-    adm, fin, res, ev, ph = S.serve(kind, N, p, steps, arr, seeds, fp, int(budget_gb * 2**30), window)
+    adm, fin, res, ev, ph = S.serve(kind, N, p, steps, arr, seeds, fp, int(budget_gb * 2**30), window,
+                                    graphs=graphs)
     wall = time.perf_counter() - t0
     lat = fin - arr
     sample = np.linspace(0, n - 1, alone_sample).round().astype(int)
@@ -335,6 +336,7 @@ def main():
     ap.add_argument("--serve-budget-gb", type=float, default=160.0)
     ap.add_argument("--serve-window", type=int, default=100)
     ap.add_argument("--serve-alone", type=int, default=6)
+    ap.add_argument("--serve-no-graphs", action="store_true", help="server rounds launch kernels one by one")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the known-size copy kernel (PMC counter calibration)")
     ap.add_argument("--traffic-summary", nargs=5, metavar=("FETCH_RUN", "WRITE_RUN", "CALIB_FETCH",
@@ -377,7 +379,7 @@ def main():
     served = None
     if not args.no_serve:
         served = run_serve(args.serve_requests, args.serve_rate, args.serve_budget_gb, args.serve_window,
-                           args.serve_alone)
+                           args.serve_alone, graphs=not args.serve_no_graphs)
 
     line = {
         "metric": "synaptic events/s (Brunel N=500k, p=0.02, deterministic rasters)",

@@ -48,14 +48,15 @@ public:
 	// budget: device bytes the admitted jobs may hold at once; window: steps per
 	// round (a job's last round may be shorter); max_delay_steps bounds every
 	// job's max_delay / dt.
-	server(int64_t budget, Int window = 100, Int max_delay_steps = 64) :
+	// graphs: rounds between admissions and retirements replay the window's
+	// launches from HIP graphs (off: every launch is issued, e.g. for tracing).
+	server(int64_t budget, Int window = 100, Int max_delay_steps = 64, bool graphs = true) :
 	_budget(budget), _window(window),
 	_multi(std::make_unique<batch>(max_delay_steps)), _single(std::make_unique<batch>(max_delay_steps)) {
 		_multi->set_window(window);
 		_single->set_window(window);
-		// rounds between admissions and retirements replay the window's launches
-		_multi->set_graphs(true);
-		_single->set_graphs(true);
+		_multi->set_graphs(graphs);
+		_single->set_graphs(graphs);
 	}
 
 	void submit(request r) { _queue.push_back(std::move(r)); }

@@ -212,7 +212,7 @@ def workloads() -> C.CDLL:
         _dp = C.POINTER(C.c_double)
         _ip = C.POINTER(_i64)
         L.wl_serve_run.argtypes = [C.c_int, C.POINTER(C.c_int), _ip, _dp, _ip, _dp, C.POINTER(C.c_uint32), _ip,
-                                   _i64, _i64, _dp, _dp, C.POINTER(_u64), _ip, _dp]
+                                   _i64, _i64, C.c_int, _dp, _dp, C.POINTER(_u64), _ip, _dp]
         L.wl_job_alone.argtypes = [C.c_int, _i64, C.c_double, _i64, C.POINTER(C.c_uint32), C.POINTER(_u64)]
         _wl = L
     return _wl
@@ -333,7 +333,7 @@ def job_footprint(kind: int, N: int, p: float, tile: int = 8192) -> int:
     return int(total * 1.1) + (1 << 20)
 
 
-def serve(kind, N, p, steps, arrival, seeds, footprint, budget, window=100):
+def serve(kind, N, p, steps, arrival, seeds, footprint, budget, window=100, graphs=True):
     """Runs jobs through spice::server (libspice_workloads.so, wl_serve_run).
     Returns (admitted, finished, result hashes, synaptic events, host seconds
     spent building / running rounds / retiring)."""
@@ -349,7 +349,7 @@ def serve(kind, N, p, steps, arrival, seeds, footprint, budget, window=100):
     r = L.wl_serve_run(n, arr(C.c_int, [int(x) for x in kind]), arr(_i64, [int(x) for x in N]),
                        arr(C.c_double, [float(x) for x in p]), arr(_i64, [int(x) for x in steps]),
                        arr(C.c_double, [float(x) for x in arrival]), sw, arr(_i64, [int(x) for x in footprint]),
-                       int(budget), int(window), adm, fin, res, C.byref(ev), ph)
+                       int(budget), int(window), int(bool(graphs)), adm, fin, res, C.byref(ev), ph)
     if r != 0:
         raise SpiceError(L.wl_error().decode())
     return np.array(adm[:]), np.array(fin[:]), [int(x) for x in res], ev.value, list(ph)

@@ -167,6 +167,8 @@ struct spice_ctx {
 	spice_proj_view* proj_views_dev = nullptr; // [projs]
 	int64_t* stateful_dev = nullptr;        // ids of projections with per-synapse state
 	int64_t nstateful = 0;
+	uint32_t* ev_sums = nullptr;            // [nstateful][ev_max_chunks] event-list scan chunk sums
+	int64_t ev_sums_cap = 0, ev_max_chunks = 1;
 	bool any_plastic = false;
 	std::vector<spice_rt::PendingFp> pending_fp;  // projections to generate
 	int64_t max_pop_size = 0;

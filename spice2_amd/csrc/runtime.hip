@@ -375,39 +375,93 @@ __global__ void __launch_bounds__(kPullThreads) deliver_pull_kernel(PullArgs a) 
 
 // Event lists of projections with per-synapse state (one-step blocks): the
 // events a target receives at the next step are laid out contiguously, target
-// by target.  One workgroup per projection scans the targets' event counts of
-// the next step (written by the counting delivery) into list offsets.
-__global__ void __launch_bounds__(1024)
-event_offsets_kernel(spice_proj_view const* pv, int64_t const* ids, ProjDev const* projs, spice_window const* win,
-                     int rel) {
-	int64_t const c       = ids[blockIdx.x];
+// by target.  The offsets are an exclusive scan of the targets' event counts of
+// the next step (written by the counting delivery), in two launches over chunks
+// of kEvChunk targets: chunk sums, then each chunk scans its counts from the
+// sum of the chunks before it.
+constexpr int kEvChunk   = 4096;
+constexpr int kEvThreads = 256;  // 16 consecutive targets per thread
+
+__device__ inline uint32_t const* next_counts(spice_proj_view const& P, ProjDev const& Q, int64_t t) {
+	return Q.counts + ((t + 1) % Q.slots) * P.dst_size;
+}
+
+__global__ void __launch_bounds__(kEvThreads)
+event_chunk_sums_kernel(spice_proj_view const* pv, int64_t const* ids, ProjDev const* projs, spice_window const* win,
+                        int rel, uint32_t* sums, int64_t max_chunks) {
+	int64_t const c         = ids[blockIdx.y];
 	spice_proj_view const P = pv[c];
-	int64_t const t       = local_step(win, P.inst, rel);
-	if (t < 0) return;
-	ProjDev const& Q       = projs[c];
-	uint32_t const* counts = Q.counts + ((t + 1) % Q.slots) * P.dst_size;
-	int64_t const D        = P.dst_size;
-	int const tid          = threadIdx.x;
-	int64_t const per      = (D + blockDim.x - 1) / blockDim.x;
-	int64_t const a = min<int64_t>(D, tid * per), b = min<int64_t>(D, a + per);
-	uint32_t sum = 0;
+	int64_t const t         = local_step(win, P.inst, rel);
+	int64_t const i0        = int64_t(blockIdx.x) * kEvChunk;
+	if (t < 0 || i0 >= P.dst_size) return;
+	uint32_t const* counts = next_counts(P, projs[c], t);
+	int64_t const a = min<int64_t>(P.dst_size, i0 + int64_t(threadIdx.x) * 16);
+	int64_t const b = min<int64_t>(P.dst_size, a + 16);
+	uint32_t sum    = 0;
 	for (int64_t i = a; i < b; i++) sum += counts[i];
-	__shared__ uint32_t scan[1024];
-	scan[tid] = sum;
+	for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 64);
+	__shared__ uint32_t ws[kEvThreads / 64];
+	if (threadIdx.x % 64 == 0) ws[threadIdx.x / 64] = sum;
 	__syncthreads();
-	for (int off = 1; off < 1024; off <<= 1) {
-		uint32_t const v = tid >= off ? scan[tid - off] : 0;
-		__syncthreads();
-		scan[tid] += v;
-		__syncthreads();
+	if (threadIdx.x == 0) {
+		uint32_t s = 0;
+		for (int w = 0; w < kEvThreads / 64; w++) s += ws[w];
+		sums[blockIdx.y * max_chunks + blockIdx.x] = s;
 	}
-	uint32_t pos = scan[tid] - sum;
-	uint32_t* offs = Q.ev_offsets;
-	for (int64_t i = a; i < b; i++) {
-		offs[i]        = pos;
-		P.ev_cursor[i] = pos;
-		pos += counts[i];
+}
+
+__global__ void __launch_bounds__(kEvThreads)
+event_offsets_kernel(spice_proj_view const* pv, int64_t const* ids, ProjDev const* projs, spice_window const* win,
+                     int rel, uint32_t const* sums, int64_t max_chunks) {
+	int64_t const c         = ids[blockIdx.y];
+	spice_proj_view const P = pv[c];
+	int64_t const t         = local_step(win, P.inst, rel);
+	int64_t const i0        = int64_t(blockIdx.x) * kEvChunk;
+	if (t < 0 || i0 >= P.dst_size) return;
+	uint32_t const* counts = next_counts(P, projs[c], t);
+	int const tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
+	constexpr int kWaves = kEvThreads / 64;
+	__shared__ uint32_t ws[kWaves];
+	__shared__ uint32_t base;
+	// events of the chunks before this one
+	uint32_t pre = 0;
+	for (int64_t q = tid; q < int64_t(blockIdx.x); q += kEvThreads) pre += sums[blockIdx.y * max_chunks + q];
+	for (int off = 32; off > 0; off >>= 1) pre += __shfl_xor(pre, off, 64);
+	if (lane == 0) ws[wave] = pre;
+	__syncthreads();
+	if (tid == 0) {
+		uint32_t s = 0;
+		for (int w = 0; w < kWaves; w++) s += ws[w];
+		base = s;
 	}
+	__syncthreads();
+	int64_t const a = min<int64_t>(P.dst_size, i0 + int64_t(tid) * 16);
+	int64_t const b = min<int64_t>(P.dst_size, a + 16);
+	uint32_t k[16];
+	uint32_t sum = 0;
+#pragma unroll
+	for (int j = 0; j < 16; j++) {
+		k[j] = a + j < b ? counts[a + j] : 0u;
+		sum += k[j];
+	}
+	uint32_t x = sum;  // inclusive scan over the wave, then over the waves
+	for (int off = 1; off < 64; off <<= 1) {
+		uint32_t const y = __shfl_up(x, off, 64);
+		if (lane >= off) x += y;
+	}
+	__syncthreads();  // every thread has read ws
+	if (lane == 63) ws[wave] = x;
+	__syncthreads();
+	uint32_t pos = base + x - sum;
+	for (int w = 0; w < wave; w++) pos += ws[w];
+	uint32_t* offs = projs[c].ev_offsets;
+#pragma unroll
+	for (int j = 0; j < 16; j++)
+		if (a + j < b) {
+			offs[a + j]        = pos;
+			P.ev_cursor[a + j] = pos;
+			pos += k[j];
+		}
 }
 
 // Appends the synapse ids of the spikes delivered this step (spikes of step
@@ -521,6 +575,16 @@ int refresh_device_descriptors(spice_ctx* ctx) {
 		ctx->any_plastic = ctx->any_plastic || Q.plastic;
 	}
 	ctx->nstateful = int64_t(stateful.size());
+	int64_t chunks = 1;  // event-list scan chunks of the largest stateful projection
+	for (int64_t c : stateful) chunks = std::max(chunks, (ctx->pops[ctx->projs[c].dst].size + kEvChunk - 1) / kEvChunk);
+	if (ctx->nstateful > 0 && ctx->nstateful * chunks > ctx->ev_sums_cap) {
+		(void)hipFree(ctx->ev_sums);
+		ctx->ev_sums     = nullptr;
+		ctx->ev_sums_cap = 0;
+		if (int r = dev_alloc(&ctx->ev_sums, size_t(ctx->nstateful * chunks))) return r;
+		ctx->ev_sums_cap = ctx->nstateful * chunks;
+	}
+	ctx->ev_max_chunks = chunks;
 	// pull tiles: every population with incoming projections, split into tiles
 	std::vector<PullTarget> pt;
 	std::vector<PullTile> tiles;
@@ -678,8 +742,11 @@ int run_blocks(spice_ctx* ctx, int32_t nsteps, int32_t L, bool timed) {
 		ctx->last_rel = rel;
 		ctx->last_n   = n;
 		if (ctx->nstateful > 0) {  // one-step blocks: event lists of the next step
-			hipLaunchKernelGGL(event_offsets_kernel, dim3(unsigned(ctx->nstateful)), dim3(1024), 0, ctx->stream,
-			                   ctx->proj_views_dev, ctx->stateful_dev, ctx->projs_dev, ctx->win_dev, rel);
+			dim3 const grid(unsigned(ctx->ev_max_chunks), unsigned(ctx->nstateful));
+			hipLaunchKernelGGL(event_chunk_sums_kernel, grid, dim3(kEvThreads), 0, ctx->stream, ctx->proj_views_dev,
+			                   ctx->stateful_dev, ctx->projs_dev, ctx->win_dev, rel, ctx->ev_sums, ctx->ev_max_chunks);
+			hipLaunchKernelGGL(event_offsets_kernel, grid, dim3(kEvThreads), 0, ctx->stream, ctx->proj_views_dev,
+			                   ctx->stateful_dev, ctx->projs_dev, ctx->win_dev, rel, ctx->ev_sums, ctx->ev_max_chunks);
 			hipLaunchKernelGGL(event_scatter_kernel, dim3(64, unsigned(ctx->nstateful)), dim3(256), 0,
 			                   ctx->stream, ctx->proj_views_dev, ctx->stateful_dev, ctx->win_dev, rel);
 			SPICE_HIP(hipGetLastError());
@@ -768,6 +835,7 @@ void spice_ctx_destroy(spice_ctx* ctx) {
 	(void)hipFree(ctx->views_dev);
 	(void)hipFree(ctx->proj_views_dev);
 	(void)hipFree(ctx->stateful_dev);
+	(void)hipFree(ctx->ev_sums);
 	(void)hipFree(ctx->pops_dev);
 	(void)hipFree(ctx->projs_dev);
 	(void)hipFree(ctx->error);

@@ -466,10 +466,10 @@ float job_delay(int kind) { return kind == 1 ? 1e-4f : 15e-4f; }
 // admitted/finished times (s) and the result hash; *events = synaptic events.
 int wl_serve_run(int n, int const* kind, int64_t const* N, double const* p, int64_t const* steps,
                  double const* arrival, uint32_t const* seeds, int64_t const* footprint, int64_t budget,
-                 int64_t window, double* admitted, double* finished, uint64_t* result, int64_t* events,
+                 int64_t window, int graphs, double* admitted, double* finished, uint64_t* result, int64_t* events,
                  double* phase_seconds) {
 	try {
-		spice::server srv(budget, window, 15);
+		spice::server srv(budget, window, 15, graphs != 0);
 		for (int i = 0; i < n; i++) {
 			spice::request r;
 			r.max_delay   = job_delay(kind[i]);
