@@ -208,6 +208,11 @@ __device__ inline void gstore64(uint64_t* p, uint64_t v) {
 	*(gptr)(p) = v;
 }
 
+__device__ inline void gstore16(uint16_t* p, uint16_t v) {
+	using gptr = __attribute__((address_space(1))) uint16_t*;
+	*(gptr)(p) = v;
+}
+
 template <class N>
 __device__ inline void load_state(N& n, uint32_t const* state, int64_t size, int64_t i) {
 	constexpr int W = int((sizeof(N) + 3) / 4);
@@ -462,9 +467,12 @@ __global__ void __launch_bounds__(256) update_lanes_kernel(spice_block b, group_
 }
 
 // Stateless neurons that draw: their spikes depend only on the step's stream,
-// so every (64-neuron chunk, step) pair is independent.  One thread per pair
-// jumps to the chunk's stream position and writes the chunk's spike word.
-// blockIdx.y = step of the block.
+// so every (16-neuron chunk, step) pair is independent.  One thread per pair
+// jumps to the chunk's stream position and writes the chunk's 16 spike bits.
+// A jump is the XOR of the step stream's states at the set bits of the
+// chunk's jump polynomial; the workgroup first builds, in LDS, the XOR of each
+// 4-bit group of states for all 16 bit patterns (32 x 16 entries from the
+// window's stored states), so a jump is 32 table reads.  blockIdx.y = step.
 template <class Neur>
 __global__ void __launch_bounds__(256) update_stateless_kernel(spice_block b, group_args<Neur> g) {
 	group_entry const e     = g.wg[blockIdx.x];
@@ -473,29 +481,50 @@ __global__ void __launch_bounds__(256) update_stateless_kernel(spice_block b, gr
 	if (u >= b.nsteps) return;
 	spice_step const st = step_of(b, v.inst, u);
 	if (st.local < 0) return;
-	int64_t const c  = e.first + threadIdx.x;
-	int64_t const i0 = c * 64;
-	if (i0 >= v.size) return;
-	Neur const neur = g.neurs[e.member];
-	device_stream rng{st.seed.s0, st.seed.s1, 0};
-	jump_stream(rng, cload_obj<spice_rng_state>(v.chunk_jump + c * 4));
-	int const n   = int(v.size - i0 < 64 ? v.size - i0 : 64);
-	uint64_t bits = 0;
-	if (n == 64) {  // whole chunk: constant shifts, one 32-bit word at a time
-		uint32_t w[2] = {0, 0};
+	__shared__ spice_rng_state tab[SPICE_STREAM_STATES / 4 * 16];
+	{
+		spice_rng_state const* S = cload_obj<spice_window>(b.win).streams +
+		                           (int64_t(v.inst) * SPICE_MAX_WINDOW + b.rel + u) * SPICE_STREAM_STATES;
+		for (int q = threadIdx.x; q < SPICE_STREAM_STATES / 4 * 16; q += blockDim.x) {
+			int const j = q >> 4, x = q & 15;
+			UInt a0 = 0, a1 = 0;
 #pragma unroll
-		for (int h = 0; h < 2; h++)
-#pragma unroll
-			for (int t = 0; t < 32; t++) w[h] |= uint32_t(call_update<Neur>(neur, nullptr, st.dt, rng)) << t;
-		bits = uint64_t(w[0]) | (uint64_t(w[1]) << 32);
-	} else {
-		for (int t = 0; t < n; t++) {
-			bool const spiked = call_update<Neur>(neur, nullptr, st.dt, rng);
-			bits |= uint64_t(spiked) << t;
+			for (int k = 0; k < 4; k++)
+				if ((x >> k) & 1) {
+					spice_rng_state const s = S[4 * j + k];
+					a0 ^= s.s0;
+					a1 ^= s.s1;
+				}
+			tab[q] = {a0, a1};
 		}
 	}
-	if (rng.count != n * v.draws) atomicExch(v.error, 1);
-	gstore64(v.bitmap + u * v.bitmap_stride + c, bits);
+	__syncthreads();
+	int64_t const c  = e.first + threadIdx.x;  // 16-neuron chunk; 4 chunks per spike word
+	int64_t const i0 = c * 16;
+	if (i0 >= (v.size + 63) / 64 * 64) return;
+	int const n   = int(v.size - i0 < 16 ? (v.size - i0 > 0 ? v.size - i0 : 0) : 16);
+	uint32_t bits = 0;
+	if (n > 0) {
+		spice_rng_state const J = cload_obj<spice_rng_state>(v.chunk_jump + c);
+		UInt a0 = 0, a1 = 0;
+#pragma unroll
+		for (int j = 0; j < 32; j++) {
+			int const x             = int(((j < 16 ? J.s0 : J.s1) >> (4 * (j & 15))) & 15);
+			spice_rng_state const t = tab[j * 16 + x];
+			a0 ^= t.s0;
+			a1 ^= t.s1;
+		}
+		device_stream rng{a0, a1, 0};
+		Neur const neur = g.neurs[e.member];
+		if (n == 16) {
+#pragma unroll
+			for (int t = 0; t < 16; t++) bits |= uint32_t(call_update<Neur>(neur, nullptr, st.dt, rng)) << t;
+		} else {
+			for (int t = 0; t < n; t++) bits |= uint32_t(call_update<Neur>(neur, nullptr, st.dt, rng)) << t;
+		}
+		if (rng.count != n * v.draws) atomicExch(v.error, 1);
+	}
+	gstore16(reinterpret_cast<uint16_t*>(v.bitmap + u * v.bitmap_stride) + c, uint16_t(bits));
 }
 
 // One thread per chunk of 16 consecutive neurons (stateful neurons that draw:

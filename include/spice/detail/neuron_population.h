@@ -225,7 +225,7 @@ template <Neuron Neur>
 class neuron_population;
 
 // Kernel shape of a population: one thread per neuron (no draws), per
-// (64-neuron chunk, step) (stateless neurons that draw), or per 16-neuron chunk.
+// (16-neuron chunk, step) (stateless neurons that draw), or per 16-neuron chunk.
 enum class update_kind { lanes, stateless, chunks };
 
 template <class Neur, class Syn, update_kind Kind, int MaxIn, bool Mixed>
@@ -247,8 +247,9 @@ struct group final : group_base {
 			auto* p = members[m];
 			nh.push_back(p->functor());
 			sh.push_back(p->slots_device());
+			// stateless: every 16-neuron chunk of the population's whole spike words
 			int64_t const elems = Kind == update_kind::lanes       ? p->size()
-			                      : Kind == update_kind::stateless ? (p->size() + 63) / 64
+			                      : Kind == update_kind::stateless ? (p->size() + 63) / 64 * 4
 			                                                       : (p->size() + 15) / 16;
 			for (int64_t f = 0; f < elems; f += per) w.push_back({int32_t(p->id()), int32_t(m), f});
 		}

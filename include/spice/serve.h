@@ -61,6 +61,13 @@ public:
 
 	void submit(request r) { _queue.push_back(std::move(r)); }
 
+	// Called after retirements, at most once per `seconds` (jobs finished, jobs
+	// submitted, seconds since run() started).
+	void on_progress(std::function<void(size_t, size_t, double)> f, double seconds = 10) {
+		_progress       = std::move(f);
+		_progress_every = seconds;
+	}
+
 	// Runs until every submitted job has finished; returns one record per job in
 	// submission order.
 	std::vector<record> run() {
@@ -80,6 +87,8 @@ public:
 			Int left;
 		};
 		std::vector<active> act;
+		size_t done        = 0;
+		double last_report = 0;
 		while (next < _queue.size() || !act.empty()) {
 			// admit every job that has arrived and fits
 			while (next < _queue.size() && _queue[order[next]].arrival <= now() &&
@@ -124,8 +133,13 @@ public:
 				act[i].b->remove(*act[i].net);
 				used -= r.footprint;
 				act.erase(act.begin() + long(i));
+				done++;
 			}
 			_t_retire += now() - tt;
+			if (_progress && now() - last_report >= _progress_every) {
+				last_report = now();
+				_progress(done, _queue.size(), last_report);
+			}
 		}
 		_queue.clear();
 		return out;
@@ -146,6 +160,8 @@ private:
 	double _t_build = 0, _t_round = 0, _t_retire = 0;
 	std::unique_ptr<batch> _multi, _single;
 	std::vector<request> _queue;
+	std::function<void(size_t, size_t, double)> _progress;
+	double _progress_every = 10;
 
 	static int64_t events(batch& b) {
 		spice_stats s{};

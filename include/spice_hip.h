@@ -112,9 +112,16 @@ typedef struct {
 
 #define SPICE_MAX_WINDOW 256 /* steps per spice_run() call */
 
-/* The steps of a window, device-resident: steps[inst * SPICE_MAX_WINDOW + k]. */
+#define SPICE_STREAM_STATES 128 /* states kept per step stream (jump tables) */
+
+/* The steps of a window, device-resident: steps[inst * SPICE_MAX_WINDOW + k].
+ * streams (null when no population draws): the first SPICE_STREAM_STATES states
+ * of every step's stream, streams[(inst * SPICE_MAX_WINDOW + k) * SPICE_STREAM_STATES + i]
+ * = the state after i outputs.  A jump by n is the XOR of the states at the set
+ * bits of x^n mod P(x) (spice_pop_view::chunk_jump holds those bits). */
 typedef struct {
 	spice_step const* steps;
+	spice_rng_state const* streams;
 	int32_t nsteps;
 	int32_t ninst;
 } spice_window;

@@ -167,6 +167,9 @@ struct spice_ctx {
 	spice_proj_view* proj_views_dev = nullptr; // [projs]
 	int64_t* stateful_dev = nullptr;        // ids of projections with per-synapse state
 	int64_t nstateful = 0;
+	bool any_draws = false;                 // a population draws: window stream states are kept
+	spice_rng_state* win_streams_dev = nullptr;  // [win_streams_cap][SPICE_MAX_WINDOW][SPICE_STREAM_STATES]
+	int64_t win_streams_cap = 0;
 	uint32_t* ev_sums = nullptr;            // [nstateful][ev_max_chunks] event-list scan chunk sums
 	int64_t ev_sums_cap = 0, ev_max_chunks = 1;
 	bool any_plastic = false;

@@ -66,6 +66,26 @@ __device__ inline int64_t local_step(spice_window const* win, int64_t inst, int 
 	return win->steps[inst * SPICE_MAX_WINDOW + rel].local;
 }
 
+// The first SPICE_STREAM_STATES states of every active step's stream of the
+// window (one thread per instance and step).
+__global__ void __launch_bounds__(64)
+stream_states_kernel(spice_step const* steps, spice_rng_state* streams, int64_t ninst, int nsteps) {
+	int64_t const q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (q >= ninst * nsteps) return;
+	int64_t const inst = q / nsteps;
+	int const k        = int(q % nsteps);
+	spice_step const st = steps[inst * SPICE_MAX_WINDOW + k];
+	if (st.local < 0) return;
+	uint64_t s0 = st.seed.s0, s1 = st.seed.s1;
+	spice_rng_state* out = streams + (inst * SPICE_MAX_WINDOW + k) * SPICE_STREAM_STATES;
+	for (int i = 0; i < SPICE_STREAM_STATES; i++) {
+		out[i]           = {s0, s1};
+		uint64_t const t = s0 ^ s1;  // xoroshiro128+ transition (rotl 24, shl 16, rotl 37)
+		s0               = ((s0 << 24) | (s0 >> 40)) ^ t ^ (t << 16);
+		s1               = (t << 37) | (t >> 27);
+	}
+}
+
 // Spike bits of one (population, step) -> ascending id list in the ring slot.
 // Each thread owns a run of consecutive bitmap words; a workgroup scan of the
 // threads' spike counts places every thread's ids.
@@ -670,7 +690,19 @@ int refresh_device_descriptors(spice_ctx* ctx) {
 		if (int r = dev_alloc(&ctx->win_steps_dev, size_t(cap) * SPICE_MAX_WINDOW)) return r;
 		ctx->win_cap = cap;
 	}
-	spice_window const w{ctx->win_steps_dev, 0, int32_t(ctx->insts.size())};
+	ctx->any_draws = false;
+	for (auto& P : ctx->pops) ctx->any_draws = ctx->any_draws || (!P.dead && P.draws > 0 && P.size > 0);
+	if (ctx->any_draws && ctx->win_streams_cap < ctx->win_cap) {
+		(void)hipFree(ctx->win_streams_dev);
+		ctx->win_streams_dev = nullptr;
+		ctx->win_streams_cap = 0;
+		if (int r = dev_alloc(&ctx->win_streams_dev, size_t(ctx->win_cap) * SPICE_MAX_WINDOW * SPICE_STREAM_STATES,
+		                      false))
+			return r;
+		ctx->win_streams_cap = ctx->win_cap;
+	}
+	spice_window const w{ctx->win_steps_dev, ctx->any_draws ? ctx->win_streams_dev : nullptr, 0,
+	                     int32_t(ctx->insts.size())};
 	if (!ctx->win_dev)
 		if (int r = dev_alloc(&ctx->win_dev, 1, false)) return r;
 	SPICE_HIP(hipMemcpy(ctx->win_dev, &w, sizeof(w), hipMemcpyHostToDevice));
@@ -831,6 +863,7 @@ void spice_ctx_destroy(spice_ctx* ctx) {
 		(void)hipEventDestroy(ctx->win_copied[b]);
 	}
 	(void)hipFree(ctx->win_steps_dev);
+	(void)hipFree(ctx->win_streams_dev);
 	(void)hipFree(ctx->win_dev);
 	(void)hipFree(ctx->views_dev);
 	(void)hipFree(ctx->proj_views_dev);
@@ -1390,6 +1423,12 @@ int spice_run(spice_ctx* ctx, int32_t nsteps, spice_step const* steps) {
 		SPICE_HIP(hipMemcpyAsync(ctx->win_steps_dev + i * SPICE_MAX_WINDOW, h + i * SPICE_MAX_WINDOW,
 		                         sizeof(spice_step) * size_t(nsteps), hipMemcpyHostToDevice, ctx->stream));
 	SPICE_HIP(hipEventRecord(ctx->win_copied[b], ctx->stream));
+	if (ctx->any_draws) {  // the window's step streams for the jump tables
+		int64_t const n = ninst * nsteps;
+		hipLaunchKernelGGL(stream_states_kernel, dim3(unsigned((n + 63) / 64)), dim3(64), 0, ctx->stream,
+		                   ctx->win_steps_dev, ctx->win_streams_dev, ninst, nsteps);
+		SPICE_HIP(hipGetLastError());
+	}
 
 	int32_t L = ctx->nstateful > 0 ? 1 : SPICE_MAX_BLOCK;  // event lists are built step by step
 	for (auto& Q : ctx->projs) L = int32_t(std::min<int64_t>(L, Q.delay));

@@ -7,6 +7,7 @@
 // 20 Hz it computes exactly what the sample's constant does).
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <exception>
 #include <memory>
@@ -470,6 +471,10 @@ int wl_serve_run(int n, int const* kind, int64_t const* N, double const* p, int6
                  double* phase_seconds) {
 	try {
 		spice::server srv(budget, window, 15, graphs != 0);
+		srv.on_progress([](size_t done, size_t total, double t) {
+			std::fprintf(stderr, "served %zu/%zu jobs after %.1f s\n", done, total, t);
+			std::fflush(stderr);
+		});
 		for (int i = 0; i < n; i++) {
 			spice::request r;
 			r.max_delay   = job_delay(kind[i]);
