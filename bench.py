@@ -265,7 +265,7 @@ def run_serve(n: int, rate: float, budget_gb: float, window: int, alone_sample: 
             "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
             "queue_wait_p50_s": float(np.percentile(adm - arr, 50)),
             "events": int(ev), "makespan_s": float(fin.max()), "wall_s": wall,
-            "host_seconds": {"build": ph[0], "rounds": ph[1], "retire": ph[2]},
+            "host_seconds": {"build": ph[0], "rounds": ph[1], "retire": ph[2], "generation_in_rounds": ph[3]},
             "value": ev / float(fin.max()), "unit": "synaptic events/s",
             "kinds": [int((kind == k).sum()) for k in range(3)],
             "sample_alone": len(sample), "sample_results_identical": bool(same),

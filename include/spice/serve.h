@@ -112,10 +112,21 @@ public:
 			// one round: every admitted job queues up to `window` steps; both
 			// classes run their windows (asynchronously, on their own streams)
 			double const tr = now();
-			for (auto& a : act) {
-				Int const n = std::min(a.left, _window);
-				for (Int k = 0; k < n; k++) a.net->step();
-				a.left -= n;
+			{
+				struct held {  // every job's steps of the round go into one window per class
+					batch* b[2];
+					held(batch* m, batch* s) : b{m, s} {
+						for (auto* x : b) x->hold(true);
+					}
+					~held() {
+						for (auto* x : b) x->hold(false);
+					}
+				} h(_multi.get(), _single.get());
+				for (auto& a : act) {
+					Int const n = std::min(a.left, _window);
+					for (Int k = 0; k < n; k++) a.net->step();
+					a.left -= n;
+				}
 			}
 			_multi->flush();
 			_single->flush();
@@ -153,6 +164,8 @@ public:
 	double build_seconds() const { return _t_build; }
 	double round_seconds() const { return _t_round; }
 	double retire_seconds() const { return _t_retire; }
+	// Host seconds spent generating admitted jobs' connectivity (inside rounds).
+	double generation_seconds() { return (csr_ms(*_multi) + csr_ms(*_single)) / 1e3; }
 
 private:
 	int64_t _budget;
@@ -163,6 +176,11 @@ private:
 	std::function<void(size_t, size_t, double)> _progress;
 	double _progress_every = 10;
 
+	static double csr_ms(batch& b) {
+		spice_stats s{};
+		detail::check_spice(spice_ctx_stats(b.context(), &s));
+		return s.csr_ms;
+	}
 	static int64_t events(batch& b) {
 		spice_stats s{};
 		detail::check_spice(spice_ctx_stats(b.context(), &s));

@@ -183,6 +183,10 @@ public:
 	}
 	// Launches every queued step (asynchronously; results are read on demand).
 	void flush() { _eng->flush(); }
+	// While held, stepping member networks only queues (no window is launched
+	// when one network's queue fills), so steps queued network by network run
+	// together in the next flush().
+	void hold(bool on) { _eng->hold = on; }
 	// Frees one network of the batch (its device memory is released; the others go on).
 	void remove(snn& net) {
 		std::erase_if(_nets, [&](std::unique_ptr<snn> const& p) { return p.get() == &net; });

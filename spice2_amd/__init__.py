@@ -336,7 +336,8 @@ def job_footprint(kind: int, N: int, p: float, tile: int = 8192) -> int:
 def serve(kind, N, p, steps, arrival, seeds, footprint, budget, window=100, graphs=True):
     """Runs jobs through spice::server (libspice_workloads.so, wl_serve_run).
     Returns (admitted, finished, result hashes, synaptic events, host seconds
-    spent building / running rounds / retiring)."""
+    spent building / running rounds / retiring / generating connectivity
+    (part of the rounds))."""
     L = workloads()
     n = len(kind)
     arr = lambda t, xs: (t * n)(*xs)
@@ -344,7 +345,7 @@ def serve(kind, N, p, steps, arrival, seeds, footprint, budget, window=100, grap
     fin = (C.c_double * n)()
     res = (_u64 * n)()
     ev = _i64()
-    ph = (C.c_double * 3)()
+    ph = (C.c_double * 4)()
     sw = (C.c_uint32 * (2 * n))(*[int(w) for s in seeds for w in s])
     r = L.wl_serve_run(n, arr(C.c_int, [int(x) for x in kind]), arr(_i64, [int(x) for x in N]),
                        arr(C.c_double, [float(x) for x in p]), arr(_i64, [int(x) for x in steps]),

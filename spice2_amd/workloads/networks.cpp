@@ -500,6 +500,7 @@ int wl_serve_run(int n, int const* kind, int64_t const* N, double const* p, int6
 		phase_seconds[0] = srv.build_seconds();
 		phase_seconds[1] = srv.round_seconds();
 		phase_seconds[2] = srv.retire_seconds();
+		phase_seconds[3] = srv.generation_seconds();
 		return 0;
 	} catch (std::exception const& e) {
 		return fail(e);
