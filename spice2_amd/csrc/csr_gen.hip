@@ -78,8 +78,32 @@ __device__ __forceinline__ xoro_state state_at(xoro_state const* sub_state, int6
 	return st;
 }
 
+// Value of lane `lane` (uniform across the wave) read into a scalar register.
+__device__ __forceinline__ uint64_t lane_u64(uint64_t v, int lane) {
+	uint32_t const lo = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(v)), lane));
+	uint32_t const hi = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(v >> 32)), lane));
+	return (uint64_t(hi) << 32) | lo;
+}
 __device__ __forceinline__ double wave_bcast(double v, int lane) {
-	return __shfl(v, lane, kWave);
+	return __builtin_bit_cast(double, lane_u64(__builtin_bit_cast(uint64_t, v), lane));
+}
+
+// Inclusive scan of a 64-bit value over the wave with DPP row shifts and row
+// broadcasts (lane l receives the sum of lanes 0..l).
+template <int Ctrl, int RowMask>
+__device__ __forceinline__ uint64_t dpp_u64(uint64_t v) {
+	uint32_t const lo = uint32_t(__builtin_amdgcn_update_dpp(0, int(uint32_t(v)), Ctrl, RowMask, 0xf, true));
+	uint32_t const hi = uint32_t(__builtin_amdgcn_update_dpp(0, int(uint32_t(v >> 32)), Ctrl, RowMask, 0xf, true));
+	return (uint64_t(hi) << 32) | lo;
+}
+__device__ __forceinline__ uint64_t wave_inclusive_scan(uint64_t x) {
+	x += dpp_u64<0x111, 0xf>(x);  // row_shr:1
+	x += dpp_u64<0x112, 0xf>(x);  // row_shr:2
+	x += dpp_u64<0x114, 0xf>(x);  // row_shr:4
+	x += dpp_u64<0x118, 0xf>(x);  // row_shr:8
+	x += dpp_u64<0x142, 0xa>(x);  // row_bcast:15 into rows 1 and 3
+	x += dpp_u64<0x143, 0xc>(x);  // row_bcast:31 into rows 2 and 3
+	return x;
 }
 
 // Phase A: generator states (every kSub positions) and per-block fixed-point
@@ -267,11 +291,7 @@ __global__ void __launch_bounds__(64) fp_chain_kernel(ChainJob* jobs) {
 			for (int i = 0; i < int(pj % kSub); i++) xoro_next(st);
 			qv = quantize(fast_draw(xoro_next(st), P.scale), P.two_F);
 		}
-		uint64_t incl = qv;  // inclusive wave scan
-		for (int off = 1; off < kWave; off <<= 1) {
-			uint64_t const t = __shfl_up(incl, off, kWave);
-			if (lane >= off) incl += t;
-		}
+		uint64_t const incl = wave_inclusive_scan(qv);
 		uint64_t const Cnext = pre(fb) + incl;  // C(pj + 1)
 		bool const valid     = pj >= s && pj <= last - 1;
 		uint64_t const G     = (Cnext - Cs) + (uint64_t(pj - s) << P.F);
@@ -287,8 +307,8 @@ __global__ void __launch_bounds__(64) fp_chain_kernel(ChainJob* jobs) {
 		}
 		// margin checks on the decisions that fix the row length
 		bool uncertain = false;
-		uint64_t const G_m  = __shfl(G, jl, kWave);
-		uint64_t const G_pm = jl > 0 ? __shfl(G, jl - 1, kWave) : 0;
+		uint64_t const G_m  = lane_u64(G, jl);
+		uint64_t const G_pm = jl > 0 ? lane_u64(G, jl - 1) : 0;
 		if (cross && m < last && G_m - P.theta < P.tol) uncertain = true;
 		if (m - 1 > s) {  // position m-2 did not cross
 			uint64_t gp;
@@ -298,7 +318,7 @@ __global__ void __launch_bounds__(64) fp_chain_kernel(ChainJob* jobs) {
 				gp = (pre(fb) - Cs) + (uint64_t((fb << 6) - 1 - s) << P.F);
 			if (!(gp < P.theta && P.theta - gp >= P.tol)) uncertain = true;
 		}
-		uint64_t Cm = __shfl(Cnext, jl, kWave);
+		uint64_t Cm = lane_u64(Cnext, jl);
 		if (uncertain) {
 			RowResult const R = row_exact(P, J.block_state, s, nullptr, 0, lds_lo, lds_hi);
 			amb += R.ambiguous;
