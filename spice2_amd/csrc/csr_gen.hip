@@ -248,12 +248,29 @@ __global__ void __launch_bounds__(64) fp_chain_kernel(ChainJob* jobs) {
 		__syncthreads();
 		wb = lo;
 		wn = min<int64_t>(kChainWin, P.nblk + 1 - lo);
-		for (int64_t i = lane; i < wn; i += kWave) wpre[i] = J.block_prefix[wb + i];
+		// eight loads per lane in flight (clamped indices: every load is issued),
+		// then their LDS writes
+		constexpr int kBurst = 8;
+		for (int64_t i0 = lane; i0 < wn; i0 += kBurst * kWave) {
+			uint64_t v[kBurst];
+#pragma unroll
+			for (int k = 0; k < kBurst; k++) v[k] = J.block_prefix[wb + min<int64_t>(i0 + k * kWave, wn - 1)];
+#pragma unroll
+			for (int k = 0; k < kBurst; k++)
+				if (i0 + k * kWave < wn) wpre[i0 + k * kWave] = v[k];
+		}
 		int64_t const ns = min<int64_t>(wn, P.nblk - wb) * (64 / kSub);
-		for (int64_t i = lane; i < ns; i += kWave) {
-			xoro_state const st = J.block_state[wb * (64 / kSub) + i];
-			ws0[i]              = st.s0;
-			ws1[i]              = st.s1;
+		for (int64_t i0 = lane; i0 < ns; i0 += kBurst * kWave) {
+			xoro_state st[kBurst];
+#pragma unroll
+			for (int k = 0; k < kBurst; k++)
+				st[k] = J.block_state[wb * (64 / kSub) + min<int64_t>(i0 + k * kWave, ns - 1)];
+#pragma unroll
+			for (int k = 0; k < kBurst; k++)
+				if (i0 + k * kWave < ns) {
+					ws0[i0 + k * kWave] = st[k].s0;
+					ws1[i0 + k * kWave] = st[k].s1;
+				}
 		}
 		__syncthreads();
 	};
