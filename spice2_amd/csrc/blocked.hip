@@ -111,8 +111,8 @@ int build_blocked(spice_ctx* ctx, Proj& Q, int64_t tile) {
 	int64_t const nrv   = (ntile + 1) * S;
 	if (nrv > (int64_t(1) << 32))
 		return set_error(SPICE_EINVAL, "pull tile too small for this projection (run table over 2^32 entries)");
-	(void)hipFree(Q.rv);
-	(void)hipFree(Q.btgt);
+	if (Q.rv) (void)hipFree(Q.rv);
+	if (Q.btgt) (void)hipFree(Q.btgt);
 	Q.rv   = nullptr;
 	Q.btgt = nullptr;
 	Q.tile = 0;
@@ -126,9 +126,11 @@ int build_blocked(spice_ctx* ctx, Proj& Q, int64_t tile) {
 		return r;
 	};
 	do {
-		if (hipMalloc(reinterpret_cast<void**>(&pos), sizeof(uint32_t) * std::max<int64_t>(nrv, 1)) != hipSuccess ||
-		    hipMalloc(reinterpret_cast<void**>(&rowv), sizeof(uint64_t) * nscan) != hipSuccess ||
-		    hipMalloc(reinterpret_cast<void**>(&tsum), sizeof(uint64_t) * ((nscan + 4095) / 4096 + 1)) != hipSuccess) {
+		// temporaries from the stream-ordered allocator (released without a device-wide wait)
+		if (hipMallocAsync(reinterpret_cast<void**>(&pos), sizeof(uint32_t) * std::max<int64_t>(nrv, 1), st) != hipSuccess ||
+		    hipMallocAsync(reinterpret_cast<void**>(&rowv), sizeof(uint64_t) * nscan, st) != hipSuccess ||
+		    hipMallocAsync(reinterpret_cast<void**>(&tsum), sizeof(uint64_t) * ((nscan + 4095) / 4096 + 1), st) !=
+		        hipSuccess) {
 			fail(SPICE_EDEVICE, "out of device memory building the pull tile view");
 			break;
 		}
@@ -164,9 +166,9 @@ int build_blocked(spice_ctx* ctx, Proj& Q, int64_t tile) {
 		Q.ntile = ntile;
 		Q.nvec  = int64_t(nvec);
 	} while (false);
-	(void)hipFree(pos);
-	(void)hipFree(rowv);
-	(void)hipFree(tsum);
+	if (pos) (void)hipFreeAsync(pos, st);
+	if (rowv) (void)hipFreeAsync(rowv, st);
+	if (tsum) (void)hipFreeAsync(tsum, st);
 	if (r) {
 		(void)hipFree(Q.rv);
 		(void)hipFree(Q.btgt);

@@ -434,13 +434,20 @@ double ulp_of(double x) {
 	return std::nextafter(x, INFINITY) - x;
 }
 
+// A temporary from the stream-ordered allocator: allocation and release are
+// ordered on the generating stream and neither waits for the rest of the device
+// (other instances keep stepping on their own streams).
 template <class T>
 struct DevBuf {
-	T* p = nullptr;
+	T* p          = nullptr;
+	hipStream_t s = nullptr;
 	~DevBuf() {
-		if (p) (void)hipFree(p);
+		if (p) (void)hipFreeAsync(p, s);
 	}
-	hipError_t alloc(size_t n) { return hipMalloc(reinterpret_cast<void**>(&p), sizeof(T) * std::max<size_t>(n, 1)); }
+	hipError_t alloc(size_t n, hipStream_t st) {
+		s = st;
+		return hipMallocAsync(reinterpret_cast<void**>(&p), sizeof(T) * std::max<size_t>(n, 1), st);
+	}
 };
 
 }  // namespace
@@ -524,7 +531,7 @@ int flush_fixed_probability(spice_ctx* ctx) {
 	ctx->pending_fp.clear();
 	std::vector<std::unique_ptr<GenJob>> jobs;
 	DevBuf<gf2poly> d_local;
-	SPICE_HIP(d_local.alloc(kGroup));
+	SPICE_HIP(d_local.alloc(kGroup, st));
 	SPICE_HIP(hipMemcpyAsync(d_local.p, local_polys().data(), sizeof(gf2poly) * kGroup, hipMemcpyHostToDevice, st));
 	std::vector<std::vector<xoro_state>> group_host;
 	// phase A
@@ -548,12 +555,12 @@ int flush_fixed_probability(spice_ctx* ctx) {
 			}
 		}
 		int64_t const ntiles = (P.nblk + 1 + 4095) / 4096;
-		SPICE_HIP(J->groups.alloc(ngroups));
-		SPICE_HIP(J->block_state.alloc(P.nblk * (64 / kSub)));
-		SPICE_HIP(J->prefix.alloc(P.nblk + 1));
-		SPICE_HIP(J->tiles.alloc(ntiles));
-		SPICE_HIP(J->total.alloc(1));
-		SPICE_HIP(J->row_start.alloc(q.S + 1));
+		SPICE_HIP(J->groups.alloc(ngroups, st));
+		SPICE_HIP(J->block_state.alloc(P.nblk * (64 / kSub), st));
+		SPICE_HIP(J->prefix.alloc(P.nblk + 1, st));
+		SPICE_HIP(J->tiles.alloc(ntiles, st));
+		SPICE_HIP(J->total.alloc(1, st));
+		SPICE_HIP(J->row_start.alloc(q.S + 1, st));
 		SPICE_HIP(hipMemcpyAsync(J->groups.p, groups.data(), sizeof(xoro_state) * ngroups, hipMemcpyHostToDevice, st));
 		SPICE_HIP(hipMemsetAsync(J->prefix.p, 0, sizeof(uint64_t) * (P.nblk + 1), st));
 		int64_t const grid = std::min<int64_t>((P.nblk + 255) / 256, 65536);
@@ -578,7 +585,7 @@ int flush_fixed_probability(spice_ctx* ctx) {
 	std::vector<int32_t> row;
 	for (size_t i = 0; i < nj; i++) cj[i] = jobs[i]->job;
 	DevBuf<ChainJob> d_jobs;
-	SPICE_HIP(d_jobs.alloc(nj));
+	SPICE_HIP(d_jobs.alloc(nj, st));
 	for (;;) {
 		SPICE_HIP(hipMemcpyAsync(d_jobs.p, cj.data(), sizeof(ChainJob) * nj, hipMemcpyHostToDevice, st));
 		hipLaunchKernelGGL(fp_chain_kernel, dim3(unsigned(nj)), dim3(64), 0, st, d_jobs.p);
@@ -627,9 +634,9 @@ int flush_fixed_probability(spice_ctx* ctx) {
 		pr.nnz = end - J.S;
 		if (pr.nnz < 0 || end > J.L) return set_error(SPICE_EDEVICE, "fixed_probability chain out of range");
 		SPICE_HIP(hipMalloc(reinterpret_cast<void**>(&pr.neighbors), sizeof(int32_t) * std::max<int64_t>(pr.nnz, 1)));
-		SPICE_HIP(J.unc.alloc(kMaxUncertain));
-		SPICE_HIP(J.nunc.alloc(1));
-		SPICE_HIP(J.amb.alloc(1));
+		SPICE_HIP(J.unc.alloc(kMaxUncertain, st));
+		SPICE_HIP(J.nunc.alloc(1, st));
+		SPICE_HIP(J.amb.alloc(1, st));
 		SPICE_HIP(hipMemsetAsync(J.nunc.p, 0, sizeof(int32_t), st));
 		SPICE_HIP(hipMemsetAsync(J.amb.p, 0, sizeof(unsigned long long), st));
 		int64_t const grid = std::min<int64_t>(J.S, 65536);

@@ -823,6 +823,13 @@ int spice_ctx_create(int device, int64_t max_delay, spice_ctx** out) {
 		delete ctx;
 		return set_error(SPICE_EHIP, hipGetErrorString(e));
 	}
+	// generation temporaries come from the device's stream-ordered pool: keep up
+	// to 8 GiB of released blocks cached for the next generation
+	hipMemPool_t pool = nullptr;
+	if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess && pool) {
+		uint64_t keep = uint64_t(8) << 30;
+		(void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+	}
 	if (int r = dev_alloc(&ctx->error, 1)) return r;
 	if (int r = dev_alloc(&ctx->events, 2)) return r;
 	*out = ctx;
