@@ -642,6 +642,30 @@ __device__ inline void catch_up_row(Syn const& f, spice_proj_view const& P, int6
 	if (lane == 0) P.ages[src] = new_age;  // every lane has read the old value above
 }
 
+// Per-synapse init() of a synapse type whose init takes no draws: every synapse
+// is independent, so one wave per source row initialises its synapses in
+// parallel (the same calls, in any order, as the host's CSR-order loop).  A
+// call that draws sets `error` (the host then reports the model unsupported).
+template <class Syn>
+__global__ void __launch_bounds__(256) init_synapses_kernel(spice_proj_view const* pv, Syn const* f, int32_t* error) {
+	using S                 = typename Syn::synapse;
+	spice_proj_view const P = *pv;
+	Syn const syn           = cload_obj<Syn>(f);
+	int const lane          = threadIdx.x % 64;
+	int64_t const gw        = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) / 64;
+	int64_t const nw        = int64_t(gridDim.x) * blockDim.x / 64;
+	for (int64_t src = gw; src < P.src_size; src += nw) {
+		int64_t const a = P.offsets[src], b = P.offsets[src + 1];
+		for (int64_t e = a + lane; e < b; e += 64) {
+			S s{};
+			device_stream rng{0, 0, 0};
+			syn.init(s, Int(src), Int(P.neighbors[e]), rng);
+			if (rng.count != 0) atomicExch(error, 1);
+			store_synapse<S>(P.syn_state, P.nnz, e, s);
+		}
+	}
+}
+
 // The sweep at steps that are multiples of 64 (snn.cpp:17-19): every source.
 // blockIdx.y: the projection (projs[y], functor funcs[y]) of a group of plastic
 // projections with one synapse type.

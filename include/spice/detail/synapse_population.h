@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <span>
+#include <stdexcept>
 #include <type_traits>
 #include <typeindex>
 #include <vector>
@@ -124,13 +125,21 @@ private:
 	Int _delay;
 	int64_t _id = -1;
 
-	// Every synapse starts as a default synapse; a per-synapse init() runs on
-	// the host in CSR order with its own generator (synapse_population.h:34-40).
+	// Every synapse starts as a default synapse; a per-synapse init() runs in
+	// CSR order with its own generator (synapse_population.h:34-40): on the
+	// device when it takes no draws (the calls are then independent), else on
+	// the host.
 	void init_synapses(engine* eng, Topology& c, util::seed_seq& seed) {
 		using S = typename Syn::synapse;
 		static_assert(std::is_trivially_copyable_v<S>, "synapse state must be trivially copyable");
 		if constexpr (PerSynapseInit<Syn>) {
 			util::xoroshiro64_128p rng(seed++);
+			if (init_draws() == 0) {
+				S const one{};
+				check_spice(spice_proj_set_synapses(eng->ctx, _id, &one, 1, int32_t(sizeof(S))));
+				init_on_device(eng);
+				return;
+			}
 			int64_t nnz = 0;
 			check_spice(spice_proj_synapses(eng->ctx, _id, &nnz));
 			std::vector<Int> off(size_t(c.src_count + 1));
@@ -145,6 +154,32 @@ private:
 			S const one{};
 			check_spice(spice_proj_set_synapses(eng->ctx, _id, &one, 1, int32_t(sizeof(S))));
 		}
+	}
+
+	// Generator outputs one init() call takes (probed on the host).
+	int init_draws() const {
+		typename Syn::synapse s{};
+		probe_stream rng;
+		_syn.init(s, 0, 0, rng);
+		return rng.count;
+	}
+
+	void init_on_device(engine* eng) {
+		spice_proj_view const* pv = nullptr;
+		check_spice(spice_ctx_proj_views(eng->ctx, &pv));
+		dev_array<Syn> f;
+		f.assign(std::vector<Syn>{_syn});
+		dev_array<int32_t> err;
+		err.assign(std::vector<int32_t>{0});
+		hipStream_t const s = eng->stream();
+		hipLaunchKernelGGL(init_synapses_kernel<Syn>, dim3(1024), dim3(256), 0, s, pv + _id, f.p, err.p);
+		check_hip(hipGetLastError(), "synapse init launch");
+		int32_t flag = 0;
+		check_hip(hipMemcpyAsync(&flag, err.p, sizeof(flag), hipMemcpyDeviceToHost, s), "synapse init");
+		check_hip(hipStreamSynchronize(s), "synapse init");
+		if (flag != 0)
+			throw std::logic_error("a synapse's init() took generator draws on the device although the host "
+			                       "probe took none; per-synapse draw counts must not depend on the synapse");
 	}
 };
 }  // namespace spice::detail
