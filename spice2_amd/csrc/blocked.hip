@@ -111,8 +111,8 @@ int build_blocked(spice_ctx* ctx, Proj& Q, int64_t tile) {
 	int64_t const nrv   = (ntile + 1) * S;
 	if (nrv > (int64_t(1) << 32))
 		return set_error(SPICE_EINVAL, "pull tile too small for this projection (run table over 2^32 entries)");
-	if (Q.rv) (void)hipFree(Q.rv);
-	if (Q.btgt) (void)hipFree(Q.btgt);
+	dfree(st, Q.rv);
+	dfree(st, Q.btgt);
 	Q.rv   = nullptr;
 	Q.btgt = nullptr;
 	Q.tile = 0;
@@ -149,8 +149,8 @@ int build_blocked(spice_ctx* ctx, Proj& Q, int64_t tile) {
 			break;
 		}
 		// one spare vector past the end: readers may load it (and never count it)
-		if (hipMalloc(reinterpret_cast<void**>(&Q.rv), sizeof(uint32_t) * std::max<int64_t>(nrv, 1)) != hipSuccess ||
-		    hipMalloc(reinterpret_cast<void**>(&Q.btgt), 16 * (nvec + 1)) != hipSuccess) {
+		if (dmalloc(reinterpret_cast<void**>(&Q.rv), sizeof(uint32_t) * std::max<int64_t>(nrv, 1)) != hipSuccess ||
+		    dmalloc(reinterpret_cast<void**>(&Q.btgt), 16 * (nvec + 1)) != hipSuccess) {
 			fail(SPICE_EDEVICE, "out of device memory building the pull tile view");
 			break;
 		}
@@ -170,8 +170,8 @@ int build_blocked(spice_ctx* ctx, Proj& Q, int64_t tile) {
 	if (rowv) (void)hipFreeAsync(rowv, st);
 	if (tsum) (void)hipFreeAsync(tsum, st);
 	if (r) {
-		(void)hipFree(Q.rv);
-		(void)hipFree(Q.btgt);
+		dfree(st, Q.rv);
+		dfree(st, Q.btgt);
 		Q.rv   = nullptr;
 		Q.btgt = nullptr;
 	}

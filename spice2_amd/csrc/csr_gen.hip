@@ -505,10 +505,10 @@ static int fp_params(int64_t S, int64_t D, double p, FpParams& P, int64_t& L) {
 
 int queue_fixed_probability(spice_ctx* ctx, int64_t proj, int64_t S, int64_t D, double p, xoro_state seed) {
 	Proj& pr = ctx->projs[proj];
-	SPICE_HIP(hipMalloc(reinterpret_cast<void**>(&pr.offsets), sizeof(int64_t) * (S + 1)));
+	SPICE_HIP(dmalloc(reinterpret_cast<void**>(&pr.offsets), sizeof(int64_t) * (S + 1)));
 	if (D == 0 || p == 0) {  // topology.cpp:84-85: all rows empty
 		SPICE_HIP(hipMemsetAsync(pr.offsets, 0, sizeof(int64_t) * (S + 1), ctx->stream));
-		SPICE_HIP(hipMalloc(reinterpret_cast<void**>(&pr.neighbors), sizeof(int32_t)));
+		SPICE_HIP(dmalloc(reinterpret_cast<void**>(&pr.neighbors), sizeof(int32_t)));
 		pr.nnz = 0;
 		return SPICE_OK;
 	}
@@ -633,7 +633,7 @@ int flush_fixed_probability(spice_ctx* ctx) {
 		SPICE_HIP(hipMemcpy(&end, J.row_start.p + J.S, sizeof(end), hipMemcpyDeviceToHost));
 		pr.nnz = end - J.S;
 		if (pr.nnz < 0 || end > J.L) return set_error(SPICE_EDEVICE, "fixed_probability chain out of range");
-		SPICE_HIP(hipMalloc(reinterpret_cast<void**>(&pr.neighbors), sizeof(int32_t) * std::max<int64_t>(pr.nnz, 1)));
+		SPICE_HIP(dmalloc(reinterpret_cast<void**>(&pr.neighbors), sizeof(int32_t) * std::max<int64_t>(pr.nnz, 1)));
 		SPICE_HIP(J.unc.alloc(kMaxUncertain, st));
 		SPICE_HIP(J.nunc.alloc(1, st));
 		SPICE_HIP(J.amb.alloc(1, st));

@@ -16,6 +16,19 @@ namespace spice_rt {
 // Sets the thread's last error and returns `code`.
 int set_error(int code, std::string const& msg);
 
+// Device memory of a context comes from the device's stream-ordered pool: it is
+// allocated in order on the null stream and waited for (so any stream may use
+// it at once), and released in order on the context's stream, so that a release
+// waits for the context's own work only, not for every stream of the device.
+inline hipError_t dmalloc(void** p, size_t bytes) {
+	hipError_t e = hipMallocAsync(p, bytes, nullptr);
+	if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+	return e;
+}
+inline void dfree(hipStream_t s, void* p) {
+	if (p) (void)hipFreeAsync(p, s);
+}
+
 // Largest pull tile: one projection's 32-bit counters for the tile must fit in
 // the LDS of a compute unit beside the run descriptors.
 constexpr int64_t kMaxPullTile = 32768;

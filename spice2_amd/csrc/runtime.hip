@@ -44,7 +44,7 @@ template <class T>
 int dev_alloc(T** p, size_t count, bool zero = true) {
 	*p = nullptr;
 	if (count == 0) count = 1;
-	hipError_t e = hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * count);
+	hipError_t e = dmalloc(reinterpret_cast<void**>(p), sizeof(T) * count);
 	if (e != hipSuccess)
 		return set_error(SPICE_ENOMEM, std::string("hipMalloc of ") + std::to_string(sizeof(T) * count) +
 		                                   " bytes failed: " + hipGetErrorString(e));
@@ -598,7 +598,7 @@ int refresh_device_descriptors(spice_ctx* ctx) {
 	int64_t chunks = 1;  // event-list scan chunks of the largest stateful projection
 	for (int64_t c : stateful) chunks = std::max(chunks, (ctx->pops[ctx->projs[c].dst].size + kEvChunk - 1) / kEvChunk);
 	if (ctx->nstateful > 0 && ctx->nstateful * chunks > ctx->ev_sums_cap) {
-		(void)hipFree(ctx->ev_sums);
+		dfree(ctx->stream, ctx->ev_sums);
 		ctx->ev_sums     = nullptr;
 		ctx->ev_sums_cap = 0;
 		if (int r = dev_alloc(&ctx->ev_sums, size_t(ctx->nstateful * chunks))) return r;
@@ -635,13 +635,13 @@ int refresh_device_descriptors(spice_ctx* ctx) {
 		pt.push_back(t);
 		for (int64_t tau = 0; tau * t.tile < P.size; tau++) tiles.push_back({id, int32_t(tau)});
 	}
-	(void)hipFree(ctx->pull_tiles);
-	(void)hipFree(ctx->pull_targets);
-	(void)hipFree(ctx->pops_dev);
-	(void)hipFree(ctx->projs_dev);
-	(void)hipFree(ctx->views_dev);
-	(void)hipFree(ctx->proj_views_dev);
-	(void)hipFree(ctx->stateful_dev);
+	dfree(ctx->stream, ctx->pull_tiles);
+	dfree(ctx->stream, ctx->pull_targets);
+	dfree(ctx->stream, ctx->pops_dev);
+	dfree(ctx->stream, ctx->projs_dev);
+	dfree(ctx->stream, ctx->views_dev);
+	dfree(ctx->stream, ctx->proj_views_dev);
+	dfree(ctx->stream, ctx->stateful_dev);
 	ctx->proj_views_dev = nullptr;
 	ctx->stateful_dev   = nullptr;
 	ctx->pull_tiles   = nullptr;
@@ -685,7 +685,7 @@ int refresh_device_descriptors(spice_ctx* ctx) {
 			SPICE_HIP(hipHostMalloc(reinterpret_cast<void**>(&ctx->win_host[b]),
 			                        sizeof(spice_step) * size_t(cap) * SPICE_MAX_WINDOW));
 		}
-		(void)hipFree(ctx->win_steps_dev);
+		dfree(ctx->stream, ctx->win_steps_dev);
 		ctx->win_steps_dev = nullptr;
 		if (int r = dev_alloc(&ctx->win_steps_dev, size_t(cap) * SPICE_MAX_WINDOW)) return r;
 		ctx->win_cap = cap;
@@ -693,7 +693,7 @@ int refresh_device_descriptors(spice_ctx* ctx) {
 	ctx->any_draws = false;
 	for (auto& P : ctx->pops) ctx->any_draws = ctx->any_draws || (!P.dead && P.draws > 0 && P.size > 0);
 	if (ctx->any_draws && ctx->win_streams_cap < ctx->win_cap) {
-		(void)hipFree(ctx->win_streams_dev);
+		dfree(ctx->stream, ctx->win_streams_dev);
 		ctx->win_streams_dev = nullptr;
 		ctx->win_streams_cap = 0;
 		if (int r = dev_alloc(&ctx->win_streams_dev, size_t(ctx->win_cap) * SPICE_MAX_WINDOW * SPICE_STREAM_STATES,
@@ -841,27 +841,27 @@ void spice_ctx_destroy(spice_ctx* ctx) {
 	(void)hipSetDevice(ctx->device);
 	(void)hipStreamSynchronize(ctx->stream);
 	for (auto& P : ctx->pops) {
-		(void)hipFree(P.state);
-		(void)hipFree(P.bitmap);
-		(void)hipFree(P.ring_ids);
-		(void)hipFree(P.ring_cnt);
-		(void)hipFree(P.chunk_jump);
-		(void)hipFree(P.hist);
+		dfree(ctx->stream, P.state);
+		dfree(ctx->stream, P.bitmap);
+		dfree(ctx->stream, P.ring_ids);
+		dfree(ctx->stream, P.ring_cnt);
+		dfree(ctx->stream, P.chunk_jump);
+		dfree(ctx->stream, P.hist);
 	}
 	for (auto& Q : ctx->projs) {
-		(void)hipFree(Q.offsets);
-		(void)hipFree(Q.neighbors);
-		(void)hipFree(Q.counts);
-		(void)hipFree(Q.rv);
-		(void)hipFree(Q.btgt);
-		(void)hipFree(Q.syn_state);
-		(void)hipFree(Q.ev_offsets);
-		(void)hipFree(Q.ev_cursor);
-		(void)hipFree(Q.events);
-		(void)hipFree(Q.ages);
+		dfree(ctx->stream, Q.offsets);
+		dfree(ctx->stream, Q.neighbors);
+		dfree(ctx->stream, Q.counts);
+		dfree(ctx->stream, Q.rv);
+		dfree(ctx->stream, Q.btgt);
+		dfree(ctx->stream, Q.syn_state);
+		dfree(ctx->stream, Q.ev_offsets);
+		dfree(ctx->stream, Q.ev_cursor);
+		dfree(ctx->stream, Q.events);
+		dfree(ctx->stream, Q.ages);
 	}
-	(void)hipFree(ctx->pull_tiles);
-	(void)hipFree(ctx->pull_targets);
+	dfree(ctx->stream, ctx->pull_tiles);
+	dfree(ctx->stream, ctx->pull_targets);
 	for (auto& t : ctx->timed)
 		for (auto e : t.ev) (void)hipEventDestroy(e);
 	drop_graphs(ctx);
@@ -869,17 +869,18 @@ void spice_ctx_destroy(spice_ctx* ctx) {
 		(void)hipHostFree(ctx->win_host[b]);
 		(void)hipEventDestroy(ctx->win_copied[b]);
 	}
-	(void)hipFree(ctx->win_steps_dev);
-	(void)hipFree(ctx->win_streams_dev);
-	(void)hipFree(ctx->win_dev);
-	(void)hipFree(ctx->views_dev);
-	(void)hipFree(ctx->proj_views_dev);
-	(void)hipFree(ctx->stateful_dev);
-	(void)hipFree(ctx->ev_sums);
-	(void)hipFree(ctx->pops_dev);
-	(void)hipFree(ctx->projs_dev);
-	(void)hipFree(ctx->error);
-	(void)hipFree(ctx->events);
+	dfree(ctx->stream, ctx->win_steps_dev);
+	dfree(ctx->stream, ctx->win_streams_dev);
+	dfree(ctx->stream, ctx->win_dev);
+	dfree(ctx->stream, ctx->views_dev);
+	dfree(ctx->stream, ctx->proj_views_dev);
+	dfree(ctx->stream, ctx->stateful_dev);
+	dfree(ctx->stream, ctx->ev_sums);
+	dfree(ctx->stream, ctx->pops_dev);
+	dfree(ctx->stream, ctx->projs_dev);
+	dfree(ctx->stream, ctx->error);
+	dfree(ctx->stream, ctx->events);
+	(void)hipStreamSynchronize(ctx->stream);  // the releases complete before the stream goes
 	(void)hipStreamDestroy(ctx->stream);
 	delete ctx;
 }
@@ -988,7 +989,7 @@ int spice_instance_destroy(spice_ctx* ctx, int64_t inst) {
 		                static_cast<void*>(Q.rv), static_cast<void*>(Q.btgt), static_cast<void*>(Q.syn_state),
 		                static_cast<void*>(Q.ev_offsets), static_cast<void*>(Q.ev_cursor),
 		                static_cast<void*>(Q.events), static_cast<void*>(Q.ages)})
-			(void)hipFree(p);
+			dfree(ctx->stream, p);
 		Proj dead;
 		dead.dead = true;
 		dead.src  = Q.src;
@@ -999,7 +1000,7 @@ int spice_instance_destroy(spice_ctx* ctx, int64_t inst) {
 		if (P.dead || P.inst != inst) continue;
 		for (void* p : {static_cast<void*>(P.state), static_cast<void*>(P.bitmap), static_cast<void*>(P.ring_ids),
 		                static_cast<void*>(P.ring_cnt), static_cast<void*>(P.chunk_jump), static_cast<void*>(P.hist)})
-			(void)hipFree(p);
+			dfree(ctx->stream, p);
 		Pop dead;
 		dead.dead = true;
 		dead.inst = P.inst;
@@ -1174,14 +1175,14 @@ int spice_proj_create_fixed_probability(spice_ctx* ctx, int64_t src, int64_t dst
 	if (int r = proj_common(ctx, src, dst, delay, Q)) return r;
 	int64_t const S = ctx->pops[src].size, D = ctx->pops[dst].size;
 	if (S == 0) {
-		(void)hipFree(Q.counts);
+		dfree(ctx->stream, Q.counts);
 		return precondition("topology.cpp", 106, "offsets.size() > src_count");
 	}
 	int64_t const id = finish_proj(ctx, Q);
 	if (int r = queue_fixed_probability(ctx, id, S, D, p, {seed_lo, seed_hi})) {
 		Proj& bad = ctx->projs.back();
-		(void)hipFree(bad.counts);
-		(void)hipFree(bad.offsets);
+		dfree(ctx->stream, bad.counts);
+		dfree(ctx->stream, bad.offsets);
 		ctx->pops[bad.dst].incoming.pop_back();
 		ctx->projs.pop_back();
 		return r;
@@ -1252,7 +1253,7 @@ int spice_proj_set_synapses(spice_ctx* ctx, int64_t proj, void const* aos, int64
 			                   dim3(256), 0, ctx->stream, Q.syn_state, Q.nnz, words, d_one);
 		SPICE_HIP(hipGetLastError());
 		SPICE_HIP(hipStreamSynchronize(ctx->stream));
-		(void)hipFree(d_one);
+		dfree(ctx->stream, d_one);
 	} else if (Q.nnz > 0) {
 		std::vector<uint32_t> soa(size_t(words) * size_t(Q.nnz), 0);
 		for (int64_t e = 0; e < Q.nnz; e++) {
