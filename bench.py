@@ -331,8 +331,7 @@ def main():
     ap.add_argument("--batch-window", type=int, default=100)
     ap.add_argument("--plain-sample", type=int, default=16)
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream leg")
-    ap.add_argument("--serve-requests", type=int, default=100,
-                    help="jobs in the served stream (the default keeps the run to a few minutes)")
+    ap.add_argument("--serve-requests", type=int, default=1000, help="jobs in the served stream")
     ap.add_argument("--serve-rate", type=float, default=40.0, help="arrivals per second")
     ap.add_argument("--serve-budget-gb", type=float, default=160.0)
     ap.add_argument("--serve-window", type=int, default=100)
