@@ -372,7 +372,6 @@ __device__ inline void update_lanes_steps(spice_block const& b, spice_pop_view c
                                           int64_t i, bool active) {
 	constexpr int A   = counts_ahead<MaxIn>;
 	constexpr bool Nv = std::is_void_v<typename neuron_traits<Neur>::type>;
-	int const lane    = threadIdx.x % 64;
 	// The block's steps are consecutive: the count slot of step u of projection
 	// c is slot0[c] + u, wrapped once (rings are longer than a block plus A).
 	uint32_t* crow[MaxIn];
@@ -437,7 +436,6 @@ __device__ inline void update_lanes_steps(spice_block const& b, spice_pop_view c
 				if (rng.count != 0) atomicExch(v.error, 1);
 			}
 			uint64_t const bits = __ballot(spiked);
-			(void)lane;
 			gstore64(v.bitmap + u * v.bitmap_stride + word, bits);
 		}
 	}
