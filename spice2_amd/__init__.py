@@ -182,6 +182,9 @@ def workloads() -> C.CDLL:
         L.wl_brunel_create.restype = _vp
         L.wl_brunel_create.argtypes = [_i64, _i64, _i64, C.c_double, C.c_float, C.c_float, C.c_float,
                                        C.c_float, C.c_float, C.POINTER(C.c_uint32), C.c_int, C.c_int]
+        L.wl_random_weight_create.restype = _vp
+        L.wl_random_weight_create.argtypes = [_i64, _i64, C.c_double, C.c_float, C.c_float, C.c_float, C.c_float,
+                                              C.POINTER(C.c_uint32), C.c_int]
         L.wl_coba_create.restype = _vp
         L.wl_coba_create.argtypes = [_i64, _i64, C.c_double, C.c_float, C.c_float, C.c_float,
                                      C.c_float, C.c_int, C.POINTER(C.c_uint32), C.c_int, C.c_int]
@@ -249,6 +252,15 @@ class Network:
         h = L.wl_brunel_plastic_create(nP, nE, nI, p, rate, w_ex, w_in, dt, delay, s, len(seeds), rule, w0,
                                        mode)
         return cls(h, (nP, nE, nI), 8)
+
+    @classmethod
+    def random_weight(cls, nP, nL, p, rate, scale, dt=1e-4, delay=15e-4, seeds=(1337,)):
+        """Poisson -> LIF over fixed_probability(p), one synapse weight per synapse
+        drawn by the synapse's init() (uniform in [0, scale))."""
+        L = workloads()
+        s = (C.c_uint32 * len(seeds))(*seeds)
+        h = L.wl_random_weight_create(nP, nL, p, rate, scale, dt, delay, s, len(seeds))
+        return cls(h, (nP, nL), 8)
 
     @classmethod
     def coba(cls, nE, nI, p, w_ex, w_in, dt=1e-4, delay=1e-4, init=1, seeds=(1337,),

@@ -107,6 +107,17 @@ struct stdp_pair {
 	}
 };
 
+// A synapse with per-synapse state whose init() draws: one uniform weight per
+// synapse from the projection's init generator, in CSR order.
+struct random_weight {
+	float scale;
+	struct synapse {
+		float w = 0;
+	};
+	void init(synapse& syn, Int, Int, auto& rng) const { syn.w = util::generate_canonical<float>(rng) * scale; }
+	void deliver(synapse const& syn, lif::neuron& to) const { to.V += syn.w; }
+};
+
 struct coba {
 	int init_mode = 0;  // 1: V ~ U[-0.06, -0.05)
 	struct neuron {
@@ -160,7 +171,7 @@ thread_local std::string g_err;
 struct network {
 	std::unique_ptr<snn> net;
 	std::vector<spice::detail::NeuronPopulation*> pops;
-	int kind = 0;  // 0 Brunel (P, E, I lif), 1 conductance (E, I)
+	int kind = 0;  // 0 Brunel (P, E, I lif), 1 conductance (E, I), 3 P -> lif with random weights
 };
 
 // Many Brunel networks in one spice::batch.
@@ -211,6 +222,26 @@ void* wl_brunel_create(int64_t nP, int64_t nE, int64_t nI, double p, float rate,
 		s.set_delivery(mode);
 		w->pops = build_brunel(s, nP, nE, nI, p, rate, wex, win, delay);
 		w->kind = 0;
+		return w.release();
+	} catch (std::exception const& e) {
+		fail(e);
+		return nullptr;
+	}
+}
+
+// P (Poisson, rate) -> L (LIF) over fixed_probability(p) with random_weight
+// synapses (their init() draws, so it runs on the host in CSR order).
+void* wl_random_weight_create(int64_t nP, int64_t nL, double p, float rate, float scale, float dt, float delay,
+                              uint32_t const* seeds, int nseeds) {
+	try {
+		auto w  = std::make_unique<network>();
+		w->net  = std::make_unique<snn>(dt, delay, make_seed(seeds, nseeds));
+		auto& s = *w->net;
+		auto P  = s.add_population<wl::poisson>(nP, {rate});
+		auto L  = s.add_population<wl::lif>(nL);
+		s.connect<wl::random_weight>(P, L, fixed_probability(p), delay, {scale});
+		w->pops = {P, L};
+		w->kind = 3;
 		return w.release();
 	} catch (std::exception const& e) {
 		fail(e);
@@ -313,8 +344,8 @@ int wl_neurons(void* h, int pop, void* out, int64_t bytes) {
 		auto* p = w->pops.at(size_t(pop));
 		void* src = p->neurons();
 		if (!src) throw std::runtime_error("stateless population");
-		int64_t const need = p->size() * (w->kind == 0 ? int64_t(sizeof(wl::lif::neuron))
-		                                               : int64_t(sizeof(wl::coba::neuron)));
+		int64_t const need = p->size() * (w->kind == 1 ? int64_t(sizeof(wl::coba::neuron))
+		                                               : int64_t(sizeof(wl::lif::neuron)));
 		if (bytes < need) throw std::runtime_error("state buffer too small");
 		std::memcpy(out, src, size_t(need));
 		return 0;
