@@ -323,6 +323,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=300)
     ap.add_argument("--tile", type=int, default=None, help="targets per pull tile")
     ap.add_argument("--no-graphs", action="store_true", help="launch the timed steps one by one")
+    ap.add_argument("--no-other-mode", action="store_true", help="skip timing the other delivery mode")
+    ap.add_argument("--other-steps", type=int, default=300, help="timed steps of the other delivery mode")
     ap.add_argument("--timing-steps", type=int, default=300,
                     help="steps after the timed region with per-kernel HIP event timing")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-sweep leg")
@@ -373,6 +375,16 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(net, 100, args.cpu_steps)
     net.close()
+    other = None
+    if not args.no_other_mode:  # the same workload with the other delivery mode
+        omode = S.SPICE_DELIVER_PULL if args.mode == "push" else S.SPICE_DELIVER_PUSH
+        onet, o = run_mode(omode, args.other_steps, args.warmup, None, graphs=not args.no_graphs,
+                           timing_steps=min(args.timing_steps, args.other_steps))
+        onet.close()
+        other = {"delivery": "pull" if args.mode == "push" else "push", "steps": args.other_steps,
+                 "value": o["events"] / o["seconds"], "ms_per_step": 1e3 * o["seconds"] / args.other_steps,
+                 "deliver_ms_per_step": o["deliver_ms"] / o["timing_steps"],
+                 "note": "same network and seed; spike rasters are identical in both modes (event counts are integers)"}
     batch = None
     if not args.no_batch:
         batch = run_batch(args.batch_instances, args.batch_steps, args.batch_window, args.plain_sample)
@@ -410,6 +422,7 @@ def main():
                      "kernel": "deliver (per launch = one block of up to 15 steps)",
                      "algorithmic_bytes_per_launch": alg_bytes / launches,
                      "avg_launch_ms": avg_ms},
+        "other_mode": other,
         "cpu_baseline": cpu,
         "batched": batch,
         "served": served,
