@@ -275,7 +275,7 @@ def run_serve(n: int, rate: float, budget_gb: float, window: int, alone_sample: 
 
 
 def run_mode(mode: int, steps: int, warmup: int, tile: int | None = None, graphs: bool = True,
-             timing_steps: int = 300):
+             timing_steps: int = 300, push_chunks: int = 0):
     """The headline run: warmup, then `steps` timed steps (windows replayed from
     HIP graphs unless graphs=False; host clock around the steps and a device
     sync), then `timing_steps` more steps launched one by one with HIP events
@@ -288,6 +288,7 @@ def run_mode(mode: int, steps: int, warmup: int, tile: int | None = None, graphs
     csr_s = time.perf_counter() - t0
     if tile:
         S.check(S.runtime().spice_ctx_set_tile(net.context(), tile))
+    S.check(S.runtime().spice_ctx_set_push_chunks(net.context(), push_chunks))
     net.set_graphs(graphs)
     net.step(warmup)
     net.sync()
@@ -324,6 +325,7 @@ def main():
     ap.add_argument("--tile", type=int, default=None, help="targets per pull tile")
     ap.add_argument("--no-graphs", action="store_true", help="launch the timed steps one by one")
     ap.add_argument("--no-other-mode", action="store_true", help="skip timing the other delivery mode")
+    ap.add_argument("--push-chunks", type=int, default=0, help="push: spike chunks per (tile, step); 0 = auto")
     ap.add_argument("--other-steps", type=int, default=300, help="timed steps of the other delivery mode")
     ap.add_argument("--timing-steps", type=int, default=300,
                     help="steps after the timed region with per-kernel HIP event timing")
@@ -359,7 +361,7 @@ def main():
     import spice2_amd as S
     mode = S.SPICE_DELIVER_PUSH if args.mode == "push" else S.SPICE_DELIVER_PULL
     net, r = run_mode(mode, args.steps, args.warmup, args.tile, graphs=not args.no_graphs,
-                      timing_steps=args.timing_steps)
+                      timing_steps=args.timing_steps, push_chunks=args.push_chunks)
     tile_used = S.runtime().spice_ctx_tile(net.context())
 
     ev = r["events"]
@@ -378,8 +380,8 @@ def main():
     other = None
     if not args.no_other_mode:  # the same workload with the other delivery mode
         omode = S.SPICE_DELIVER_PULL if args.mode == "push" else S.SPICE_DELIVER_PUSH
-        onet, o = run_mode(omode, args.other_steps, args.warmup, None, graphs=not args.no_graphs,
-                           timing_steps=min(args.timing_steps, args.other_steps))
+        onet, o = run_mode(omode, args.other_steps, args.warmup, args.tile, graphs=not args.no_graphs,
+                           timing_steps=min(args.timing_steps, args.other_steps), push_chunks=args.push_chunks)
         onet.close()
         other = {"delivery": "pull" if args.mode == "push" else "push", "steps": args.other_steps,
                  "value": o["events"] / o["seconds"], "ms_per_step": 1e3 * o["seconds"] / args.other_steps,

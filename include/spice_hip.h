@@ -38,7 +38,7 @@ extern "C" {
  * event counts and the neuron kernel replays deliver() count times in connect
  * order (the order the reference applies them, synapse_population.h:88-134). */
 #define SPICE_DELIVER_PULL 0 /* target-tile owned, LDS counters, no global atomics */
-#define SPICE_DELIVER_PUSH 1 /* source-row parallel, global atomic counters */
+#define SPICE_DELIVER_PUSH 1 /* spike chunks counted in LDS, then global atomic counters */
 
 typedef struct spice_ctx spice_ctx;
 
@@ -169,7 +169,10 @@ int spice_ctx_create(int device, int64_t max_delay, spice_ctx** out);
 void spice_ctx_destroy(spice_ctx* ctx);
 /* Delivery algorithm (default SPICE_DELIVER_PULL); fixed once steps have run. */
 int spice_ctx_set_delivery(spice_ctx* ctx, int mode);
-/* Targets per pull tile (multiple of 64, <= 32768; LDS use is 4 bytes per target and
+/* Spike chunks per (tile, step) of push delivery, 1..64 (0, the default: enough
+ * workgroups for one per compute unit, at most 8).  Any value gives the same counts. */
+int spice_ctx_set_push_chunks(spice_ctx* ctx, int32_t chunks);
+/* Targets per delivery tile (multiple of 64, <= 32768; LDS use is 4 bytes per target and
  * projection counted together). */
 int spice_ctx_set_tile(spice_ctx* ctx, int64_t tile);
 int64_t spice_ctx_tile(spice_ctx* ctx);

@@ -66,6 +66,7 @@ def runtime() -> C.CDLL:
         L.spice_ctx_set_delivery.argtypes = [_vp, C.c_int]
         L.spice_ctx_set_timing.argtypes = [_vp, C.c_int]
         L.spice_ctx_set_tile.argtypes = [_vp, _i64]
+        L.spice_ctx_set_push_chunks.argtypes = [_vp, C.c_int32]
         L.spice_ctx_tile.restype = _i64
         L.spice_ctx_tile.argtypes = [_vp]
         L.spice_ctx_stream.restype = _vp

@@ -154,46 +154,15 @@ __global__ void __launch_bounds__(256) history_kernel(PopDev const* pops, spice_
 	P.hist[i] = h;
 }
 
-// Push delivery: one workgroup row per (projection, step); its waves walk the
-// step's spikes, each wave streaming one spike's CSR row and incrementing the
-// target's event counter for the application step with a device-scope atomic.
-// Counts are integers, so the result does not depend on the order of the
-// atomics.  blockIdx.y = projection * SPICE_MAX_BLOCK + step.
-constexpr int kPushRowGroups = 16;
-__global__ void __launch_bounds__(256) deliver_push_kernel(ProjDev const* projs, spice_window const* win,
-                                                           int rel, int nsteps, int64_t ring,
-                                                           unsigned long long* events) {
-	int const k = int(blockIdx.y % SPICE_MAX_BLOCK);
-	if (k >= nsteps) return;
-	ProjDev const& P = projs[blockIdx.y / SPICE_MAX_BLOCK];
-	if (P.inst < 0) return;  // destroyed
-	int64_t const t  = local_step(win, P.inst, rel + k);
-	if (t < 0) return;
-	int64_t const slot = t % ring;
-	int64_t const nsp  = P.src_ring_cnt[slot];
-	int32_t const* ids = P.src_ring_ids + slot * P.src_size;
-	uint32_t* cnt      = P.counts + ((t + P.delay) % P.slots) * P.dst_size;
-	int const lane     = threadIdx.x % kWave;
-	int64_t const gw   = int64_t(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
-	int64_t const nw   = int64_t(gridDim.x) * (blockDim.x / kWave);
-	unsigned long long ev = 0, rows = 0;
-	for (int64_t idx = gw; idx < nsp; idx += nw) {
-		int32_t const src = ids[idx];
-		int64_t const a = P.offsets[src], b = P.offsets[src + 1];
-		for (int64_t e = a + lane; e < b; e += kWave) atomicAdd(cnt + P.neighbors[e], 1u);
-		ev += uint64_t(b - a);
-		rows++;
-	}
-	if (lane == 0 && rows) {
-		atomicAdd(events, ev);
-		atomicAdd(events + 1, rows);
-	}
-}
-
-// Pull delivery: one workgroup per (target tile, step).  The workgroup counts,
-// in LDS, the events that the step's spikes of its incoming projections put on
-// each target of its tile, then writes the tile's counters once: no global
-// atomics.  Counters are 32 bits (a target receives at most one event per
+// Tiled delivery.  Pull: one workgroup per (target tile, step).  The workgroup
+// counts, in LDS, the events that the step's spikes of its incoming projections
+// put on each target of its tile, then writes the tile's counters once: no
+// global atomics.  Push (shared-then-global): one workgroup per (target tile,
+// step, chunk of the step's spikes); it counts its chunk's events in LDS the
+// same way and adds its nonzero counters to the global ones with device-scope
+// atomics, in tile order (16 counters of one cache line per request), so a
+// launch issues about one global atomic request per 16 targets and chunk
+// instead of one per event; the neuron update clears the counters it consumes.  Counters are 32 bits (a target receives at most one event per
 // spiking source of a projection); the padding slots of the runs count into
 // kPadSlots spare counters past the tile, which are never written out.  Counts
 // are integers, so the result does not depend on the order of the LDS atomics
@@ -211,7 +180,7 @@ __global__ void __launch_bounds__(256) deliver_push_kernel(ProjDev const* projs,
 // descriptors of a batch of spikes (address of the first vector, vector count,
 // counter base) are computed first, one spike per thread, and staged in LDS.
 //
-// Workgroup L handles tile L / 16 and step L % 16: the steps of one block land on
+// Workgroup L handles step L % 16 and (tile, chunk) L / 16: the steps of one block land on
 // fixed XCDs (workgroups are dealt to the 8 XCDs round-robin), so the spike
 // lists and run bounds that all workgroups of a step read are cached in one L2.
 struct PullArgs {
@@ -224,6 +193,7 @@ struct PullArgs {
 	int64_t ring;
 	int64_t max_tile;  // largest tile of the launch (sets the LDS layout)
 	int32_t pass;      // projections counted together (<= kPassProj)
+	int32_t chunks;    // push: spike chunks per (tile, step)
 	unsigned long long* events;
 };
 
@@ -286,11 +256,14 @@ int pull_pass(int64_t tile, int max_in) {
 	return pull_lds_bytes(tile, p) <= kLdsMax ? p : 0;
 }
 
-__global__ void __launch_bounds__(kPullThreads) deliver_pull_kernel(PullArgs a) {
+template <bool Push>
+__device__ __forceinline__ void deliver_tiles(PullArgs const& a) {
 	extern __shared__ uint32_t lds[];  // [pass][max_tile + kPadSlots] counters, then descriptors
 	int const k = int(blockIdx.x % SPICE_MAX_BLOCK);
 	if (k >= a.nsteps) return;
-	PullTile const pt    = a.tiles[blockIdx.x / SPICE_MAX_BLOCK];
+	unsigned const tc    = blockIdx.x / SPICE_MAX_BLOCK;
+	int const chunk      = Push ? int(tc % unsigned(a.chunks)) : 0;
+	PullTile const pt    = a.tiles[Push ? tc / unsigned(a.chunks) : tc];
 	PullTarget const& tg = a.targets[pt.target];
 	int64_t const step   = local_step(a.win, tg.inst, a.rel + k);
 	if (step < 0) return;
@@ -318,11 +291,14 @@ __global__ void __launch_bounds__(kPullThreads) deliver_pull_kernel(PullArgs a) 
 		pre[0] = 0;
 #pragma unroll
 		for (int j = 0; j < kPassProj; j++) pre[j + 1] = pre[j] + (j < np ? pp[j0 + j].nsp : 0);
-		int64_t const M = pre[kPassProj];
-		if (pt.tau == 0 && tid == 0) rows += uint64_t(M);
+		int64_t const M  = pre[kPassProj];
+		int64_t const lo = Push ? M * chunk / a.chunks : 0;  // this workgroup's spikes of the pass
+		int64_t const hi = Push ? M * (chunk + 1) / a.chunks : M;
+		if (pt.tau == 0 && tid == 0) rows += uint64_t(hi - lo);
+		if (Push && lo == hi) continue;  // nothing to add (uniform over the workgroup)
 		for (int i = tid; i < np * stride; i += kPullThreads) lds[i] = 0;
-		for (int64_t d0 = 0; d0 < M; d0 += kDescBatch) {
-			int const nd = int(min<int64_t>(kDescBatch, M - d0));
+		for (int64_t d0 = lo; d0 < hi; d0 += kDescBatch) {
+			int const nd = int(min<int64_t>(kDescBatch, hi - d0));
 			__syncthreads();  // counters cleared / previous descriptors consumed
 			if (tid < nd) {
 				int64_t const g = d0 + tid;
@@ -384,7 +360,14 @@ __global__ void __launch_bounds__(kPullThreads) deliver_pull_kernel(PullArgs a) 
 		__syncthreads();
 		for (int j = 0; j < np; j++) {
 			uint32_t* out = pp[j0 + j].out;
-			for (int i = tid; i < Tn; i += kPullThreads) gstore(out + i, lds[j * stride + i]);
+			if (Push) {
+				for (int i = tid; i < Tn; i += kPullThreads) {
+					uint32_t const c = lds[j * stride + i];
+					if (c) atomicAdd(out + i, c);
+				}
+			} else {
+				for (int i = tid; i < Tn; i += kPullThreads) gstore(out + i, lds[j * stride + i]);
+			}
 		}
 		__syncthreads();  // counters written before the next pass clears them
 	}
@@ -392,6 +375,9 @@ __global__ void __launch_bounds__(kPullThreads) deliver_pull_kernel(PullArgs a) 
 	if (lane == 0 && ev) atomicAdd(a.events, ev);
 	if (tid == 0 && rows) atomicAdd(a.events + 1, rows);
 }
+
+__global__ void __launch_bounds__(kPullThreads) deliver_pull_kernel(PullArgs a) { deliver_tiles<false>(a); }
+__global__ void __launch_bounds__(kPullThreads) deliver_push_kernel(PullArgs a) { deliver_tiles<true>(a); }
 
 // Event lists of projections with per-synapse state (one-step blocks): the
 // events a target receives at the next step are laid out contiguously, target
@@ -522,13 +508,11 @@ int refresh_device_descriptors(spice_ctx* ctx) {
 	if (!ctx->dev_dirty) return SPICE_OK;
 	drop_graphs(ctx);
 	SPICE_HIP(hipStreamSynchronize(ctx->stream));
-	if (ctx->mode == SPICE_DELIVER_PULL) {
-		for (auto& Q : ctx->projs) {
-			if (Q.dead) continue;
-			int64_t const t = pop_tile(ctx, ctx->pops[Q.dst]);
-			if (Q.tile != t)
-				if (int r = build_blocked(ctx, Q, t)) return r;
-		}
+	for (auto& Q : ctx->projs) {  // both modes deliver over the tile view
+		if (Q.dead) continue;
+		int64_t const t = pop_tile(ctx, ctx->pops[Q.dst]);
+		if (Q.tile != t)
+			if (int r = build_blocked(ctx, Q, t)) return r;
 	}
 	std::vector<PopDev> pd;
 	std::vector<spice_pop_view> views;
@@ -616,13 +600,11 @@ int refresh_device_descriptors(spice_ctx* ctx) {
 		if (!P.incoming.empty()) ctx->pull_max_tile = std::max(ctx->pull_max_tile, pop_tile(ctx, P));
 	}
 	ctx->pull_pass = pull_pass(ctx->pull_max_tile, max_in);
-	if (ctx->mode == SPICE_DELIVER_PULL && ctx->pull_pass == 0)
-		return set_error(SPICE_EINVAL, "pull tile too large for the LDS of a compute unit");
-	if (ctx->mode == SPICE_DELIVER_PULL) {
-		SPICE_HIP(hipFuncSetAttribute(reinterpret_cast<void const*>(&deliver_pull_kernel),
-		                              hipFuncAttributeMaxDynamicSharedMemorySize,
-		                              int(pull_lds_bytes(ctx->pull_max_tile, ctx->pull_pass))));
-	}
+	if (ctx->pull_pass == 0) return set_error(SPICE_EINVAL, "tile too large for the LDS of a compute unit");
+	void const* kern = ctx->mode == SPICE_DELIVER_PUSH ? reinterpret_cast<void const*>(&deliver_push_kernel)
+	                                                   : reinterpret_cast<void const*>(&deliver_pull_kernel);
+	SPICE_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+	                              int(pull_lds_bytes(ctx->pull_max_tile, ctx->pull_pass))));
 	for (auto& P : ctx->pops) {
 		if (P.dead || P.incoming.empty() || P.size == 0) continue;
 		PullTarget t{};
@@ -724,21 +706,31 @@ int check_device_error(spice_ctx* ctx) {
 	return set_error(SPICE_EDEVICE, "device-side check failed (code " + std::to_string(flag) + ")");
 }
 
+// Spike chunks per (tile, step) of a push launch of n steps: the context's
+// setting, else enough workgroups for one per compute unit (at most 8 chunks).
+// Every chunk past the first adds a global atomic per counter it touches, so
+// networks that already give every compute unit a (tile, step) run unsplit.
+int32_t push_chunks(spice_ctx const* ctx, int32_t n) {
+	if (ctx->push_chunks > 0) return ctx->push_chunks;
+	int64_t const wgs   = std::max<int64_t>(ctx->pull_ntiles * n, 1);
+	int64_t const slots = int64_t(ctx->cus);
+	return int32_t(std::min<int64_t>(8, std::max<int64_t>(1, (slots + wgs - 1) / wgs)));
+}
+
 // Delivers the spikes of steps rel .. rel+n-1 of the staged window.
 int launch_delivery(spice_ctx* ctx, int32_t rel, int32_t n) {
-	if (!ctx->projs.empty() && ctx->mode == SPICE_DELIVER_PUSH) {
-		unsigned const rowsy = unsigned(ctx->projs.size() * SPICE_MAX_BLOCK);
-		hipLaunchKernelGGL(deliver_push_kernel, dim3(kPushRowGroups, rowsy), dim3(256), 0, ctx->stream,
-		                   ctx->projs_dev, ctx->win_dev, rel, n, ctx->ring, ctx->events);
-		SPICE_HIP(hipGetLastError());
-	} else if (!ctx->projs.empty() && ctx->pull_ntiles > 0) {
-		PullArgs a{ctx->projs_dev, ctx->pull_tiles, ctx->pull_targets, ctx->win_dev, rel, n,
-		           ctx->ring, ctx->pull_max_tile, ctx->pull_pass, ctx->events};
-		hipLaunchKernelGGL(deliver_pull_kernel, dim3(unsigned(ctx->pull_ntiles * SPICE_MAX_BLOCK)),
-		                   dim3(kPullThreads), pull_lds_bytes(ctx->pull_max_tile, ctx->pull_pass),
-		                   ctx->stream, a);
-		SPICE_HIP(hipGetLastError());
-	}
+	if (ctx->projs.empty() || ctx->pull_ntiles == 0) return SPICE_OK;
+	bool const push    = ctx->mode == SPICE_DELIVER_PUSH;
+	int32_t const ch   = push ? push_chunks(ctx, n) : 1;
+	PullArgs const a{ctx->projs_dev, ctx->pull_tiles, ctx->pull_targets, ctx->win_dev, rel, n,
+	                 ctx->ring, ctx->pull_max_tile, ctx->pull_pass, ch, ctx->events};
+	dim3 const grid(unsigned(ctx->pull_ntiles * ch * SPICE_MAX_BLOCK));
+	size_t const lds = pull_lds_bytes(ctx->pull_max_tile, ctx->pull_pass);
+	if (push)
+		hipLaunchKernelGGL(deliver_push_kernel, grid, dim3(kPullThreads), lds, ctx->stream, a);
+	else
+		hipLaunchKernelGGL(deliver_pull_kernel, grid, dim3(kPullThreads), lds, ctx->stream, a);
+	SPICE_HIP(hipGetLastError());
 	return SPICE_OK;
 }
 
@@ -830,6 +822,9 @@ int spice_ctx_create(int device, int64_t max_delay, spice_ctx** out) {
 		uint64_t keep = uint64_t(8) << 30;
 		(void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
 	}
+	int cus = 0;
+	if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+		ctx->cus = cus;
 	if (int r = dev_alloc(&ctx->error, 1)) return r;
 	if (int r = dev_alloc(&ctx->events, 2)) return r;
 	*out = ctx;
@@ -894,6 +889,13 @@ int spice_ctx_set_delivery(spice_ctx* ctx, int mode) {
 		return set_error(SPICE_EINVAL, "the delivery mode can only be changed before the first step");
 	ctx->mode      = mode;
 	ctx->dev_dirty = true;
+	return SPICE_OK;
+}
+
+int spice_ctx_set_push_chunks(spice_ctx* ctx, int32_t chunks) {
+	if (chunks < 0 || chunks > 64) return set_error(SPICE_EINVAL, "push chunks must be in [0, 64]");
+	if (ctx->push_chunks != chunks) drop_graphs(ctx);  // captured launches hold the grid
+	ctx->push_chunks = chunks;
 	return SPICE_OK;
 }
 

@@ -156,6 +156,29 @@ def test_pull_tile_sizes(tile):
             assert cnt == z["counts"][t].tolist() and h == int(z["raster_hash"][t]), (tile, t)
 
 
+@pytest.mark.parametrize("chunks,tile", [(1, 8192), (3, 640), (64, 64)])
+def test_push_chunks(chunks, tile):
+    """Push delivery gives the reference raster for any number of spike chunks
+    per (tile, step), including chunks left without spikes (64 chunks)."""
+    rec = NETS["sweep1"]
+    z = np.load(G / "brunel_sweep1.npz")
+    print("push chunks", chunks, "tile", tile, flush=True)
+    with brunel(rec, S.SPICE_DELIVER_PUSH) as net:
+        S.check(S.runtime().spice_ctx_set_tile(net.context(), tile))
+        S.check(S.runtime().spice_ctx_set_push_chunks(net.context(), chunks))
+        for t in range(120):
+            net.step()
+            cnt, h, _ = step_record(net)
+            assert cnt == z["counts"][t].tolist() and h == int(z["raster_hash"][t]), (chunks, t)
+
+
+def test_push_chunks_range():
+    rec = NETS["sweep0"]
+    with brunel(rec, S.SPICE_DELIVER_PUSH) as net:
+        with pytest.raises(S.SpiceError, match="push chunks"):
+            S.check(S.runtime().spice_ctx_set_push_chunks(net.context(), 65))
+
+
 def test_mode_is_fixed_after_first_step():
     rec = NETS["sweep0"]
     with brunel(rec) as net:
