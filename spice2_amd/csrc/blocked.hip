@@ -1,4 +1,4 @@
-// Tile view of a projection's CSR for pull delivery.
+// Tile view of a projection's CSR for delivery (pull and push).
 //
 // Targets are split into tiles of `tile` consecutive neurons.  Rows are sorted,
 // so a row's targets in one tile form a contiguous run.  Kept beside the
@@ -106,11 +106,11 @@ int build_blocked(spice_ctx* ctx, Proj& Q, int64_t tile) {
 	int64_t const S      = ctx->pops[Q.src].size;
 	int64_t const D      = ctx->pops[Q.dst].size;
 	if (tile <= 0 || tile > kMaxPullTile || tile % 64)
-		return set_error(SPICE_EINVAL, "pull tile must be a multiple of 64 in [64, " + std::to_string(kMaxPullTile) + "]");
+		return set_error(SPICE_EINVAL, "tile must be a multiple of 64 in [64, " + std::to_string(kMaxPullTile) + "]");
 	int64_t const ntile = std::max<int64_t>((D + tile - 1) / tile, 1);
 	int64_t const nrv   = (ntile + 1) * S;
 	if (nrv > (int64_t(1) << 32))
-		return set_error(SPICE_EINVAL, "pull tile too small for this projection (run table over 2^32 entries)");
+		return set_error(SPICE_EINVAL, "tile too small for this projection (run table over 2^32 entries)");
 	dfree(st, Q.rv);
 	dfree(st, Q.btgt);
 	Q.rv   = nullptr;
@@ -131,7 +131,7 @@ int build_blocked(spice_ctx* ctx, Proj& Q, int64_t tile) {
 		    hipMallocAsync(reinterpret_cast<void**>(&rowv), sizeof(uint64_t) * nscan, st) != hipSuccess ||
 		    hipMallocAsync(reinterpret_cast<void**>(&tsum), sizeof(uint64_t) * ((nscan + 4095) / 4096 + 1), st) !=
 		        hipSuccess) {
-			fail(SPICE_EDEVICE, "out of device memory building the pull tile view");
+			fail(SPICE_EDEVICE, "out of device memory building the tile view");
 			break;
 		}
 		SPICE_HIP(hipMemsetAsync(rowv, 0, sizeof(uint64_t) * nscan, st));
@@ -145,13 +145,13 @@ int build_blocked(spice_ctx* ctx, Proj& Q, int64_t tile) {
 		SPICE_HIP(hipMemcpyAsync(&nvec, rowv + S, sizeof(nvec), hipMemcpyDeviceToHost, st));
 		SPICE_HIP(hipStreamSynchronize(st));
 		if (nvec >= (uint64_t(1) << 32) - 1) {
-			fail(SPICE_EINVAL, "projection too large for pull delivery (over 2^32 target vectors); use push");
+			fail(SPICE_EINVAL, "projection too large for tiled delivery (over 2^32 16-byte target vectors)");
 			break;
 		}
 		// one spare vector past the end: readers may load it (and never count it)
 		if (dmalloc(reinterpret_cast<void**>(&Q.rv), sizeof(uint32_t) * std::max<int64_t>(nrv, 1)) != hipSuccess ||
 		    dmalloc(reinterpret_cast<void**>(&Q.btgt), 16 * (nvec + 1)) != hipSuccess) {
-			fail(SPICE_EDEVICE, "out of device memory building the pull tile view");
+			fail(SPICE_EDEVICE, "out of device memory building the tile view");
 			break;
 		}
 		SPICE_HIP(hipMemsetAsync(Q.btgt + nvec * 8, 0, 16, st));
