@@ -274,6 +274,18 @@ def run_serve(n: int, rate: float, budget_gb: float, window: int, alone_sample: 
                    "Poisson arrivals; numpy PCG64(2025)"}
 
 
+def step_roofline(events: int, rows: int, steps: int, value: float) -> dict:
+    """Whole-step roofline of the Brunel workload: algorithmic bytes per step =
+    4 B per synaptic event (int32 target) + 8 B per delivered row (CSR offsets)
+    + the LIF state read and written once (2 x 8 B per E/I neuron; Poisson
+    neurons have no state).  At 8 TB/s this bounds events/s; `frac` is the
+    measured rate over that bound."""
+    per_step = (4.0 * events + 8.0 * rows) / steps + 2 * 8.0 * (NE + NI)
+    bound = (events / steps) / (per_step / (HBM_PEAK_GBS * 1e9))
+    return {"bytes_per_step": per_step, "roofline_events_per_s": bound, "frac": value / bound,
+            "note": "whole step (update + compaction + delivery) against HBM peak"}
+
+
 def run_mode(mode: int, steps: int, warmup: int, tile: int | None = None, graphs: bool = True,
              timing_steps: int = 300, push_chunks: int = 0):
     """The headline run: warmup, then `steps` timed steps (windows replayed from
@@ -424,6 +436,7 @@ def main():
                      "kernel": "deliver (per launch = one block of up to 15 steps)",
                      "algorithmic_bytes_per_launch": alg_bytes / launches,
                      "avg_launch_ms": avg_ms},
+        "step_roofline": step_roofline(ev, r["rows"], args.steps, value),
         "other_mode": other,
         "cpu_baseline": cpu,
         "batched": batch,
