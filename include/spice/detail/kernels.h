@@ -522,7 +522,8 @@ __global__ void __launch_bounds__(256) update_stateless_kernel(spice_block b, gr
 		}
 		if (rng.count != n * v.draws) atomicExch(v.error, 1);
 	}
-	gstore16(reinterpret_cast<uint16_t*>(v.bitmap + u * v.bitmap_stride) + c, uint16_t(bits));
+	// one bitmap row per step of the window (launched once per window, rel 0)
+	gstore16(reinterpret_cast<uint16_t*>(v.bitmap + (b.rel + u) * v.bitmap_stride) + c, uint16_t(bits));
 }
 
 // One thread per chunk of 16 consecutive neurons (stateful neurons that draw:

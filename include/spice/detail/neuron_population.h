@@ -60,6 +60,8 @@ struct group_base {
 	virtual void add(NeuronPopulation* p)                        = 0;
 	virtual void rebuild()                                       = 0;
 	virtual void launch(spice_block const& b, hipStream_t s)     = 0;
+	// updated once per window, every step at once (stateless populations that draw)
+	virtual bool per_window() const                              = 0;
 };
 
 // The plastic projections of one synapse type (in every instance): one sweep
@@ -127,9 +129,11 @@ struct engine {
 	static int launch_all(void* user, spice_block const* b, void* stream) {
 		auto* e = static_cast<engine*>(user);
 		try {
-			if (b->phase == SPICE_PHASE_NEURONS)
-				for (auto& g : e->groups) g.second->launch(*b, static_cast<hipStream_t>(stream));
-			else
+			if (b->phase == SPICE_PHASE_NEURONS || b->phase == SPICE_PHASE_WINDOW) {
+				bool const window = b->phase == SPICE_PHASE_WINDOW;
+				for (auto& g : e->groups)
+					if (g.second->per_window() == window) g.second->launch(*b, static_cast<hipStream_t>(stream));
+			} else
 				for (auto& g : e->plastic_groups) g.second->launch(*b, static_cast<hipStream_t>(stream));
 		} catch (std::exception const& x) {
 			e->cb_error = x.what();
@@ -258,6 +262,8 @@ struct group final : group_base {
 		slots.assign(sh);
 		nwg = unsigned(w.size());
 	}
+
+	bool per_window() const override { return Kind == update_kind::stateless; }
 
 	void launch(spice_block const& b, hipStream_t s) override {
 		if (nwg == 0) return;

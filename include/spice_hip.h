@@ -67,8 +67,10 @@ typedef struct {
 	int64_t size;
 	int32_t words;          /* 32-bit words per neuron (sizeof(neuron) rounded up) */
 	int32_t draws;          /* RNG outputs each neuron takes per step (stream offset stride) */
-	uint64_t* bitmap;       /* [SPICE_MAX_BLOCK][bitmap_stride] spike bits of the running block;
-	                           the last word of a row is scratch (update kernels' waves past the end) */
+	uint64_t* bitmap;       /* spike bits, row per step: [SPICE_MAX_BLOCK] rows indexed by the
+	                           step in the running block, or for stateless populations that draw
+	                           (no state words) [SPICE_MAX_WINDOW] rows indexed by the step in the
+	                           window; the last word of a row is scratch (waves past the end) */
 	int64_t bitmap_stride;
 	/* per chunk of 16 neurons: x^(stream offset of the chunk's first draw) mod P,
 	 * P the generator's characteristic polynomial (offset counted from the
@@ -134,11 +136,13 @@ typedef struct {
 	spice_proj_view const* projs;
 	int32_t rel;
 	int32_t nsteps;
-	int32_t phase; /* SPICE_PHASE_NEURONS or SPICE_PHASE_SYNAPSES */
+	int32_t phase; /* SPICE_PHASE_WINDOW, SPICE_PHASE_NEURONS or SPICE_PHASE_SYNAPSES */
 } spice_block;
 
 #define SPICE_PHASE_NEURONS 0  /* neuron updates (first in a block) */
 #define SPICE_PHASE_SYNAPSES 1 /* plastic-synapse sweep and deliveries (last in a block) */
+#define SPICE_PHASE_WINDOW 2   /* stateless populations that draw, every step of the window at
+                                  once (first in a window; rel = 0, nsteps = the window's) */
 
 /* Launches the caller's kernels of one phase of one block on `stream` (a
  * hipStream_t), without synchronising or allocating: blocks may be captured
@@ -260,11 +264,14 @@ int64_t spice_instance_count(spice_ctx* ctx);
  * spice_run() runs a window of nsteps <= SPICE_MAX_WINDOW steps.  steps is
  * [instance][nsteps]: for each instance and step, its dt and stream seed, and
  * whether it steps (an instance's active steps within a window form a prefix).
- * The window is split into blocks of at most min(SPICE_MAX_BLOCK, smallest
- * projection delay) steps; per block:
- *   1. the update callback (SPICE_PHASE_NEURONS) launches every population's
- *      neuron-update kernel (each applies the deliveries due, updates, and
- *      sets spike bits);
+ * First the update callback (SPICE_PHASE_WINDOW) launches the update of the
+ * stateless populations that draw (no state words: their spikes depend only on
+ * the steps' streams) for every step of the window.  The window is then split
+ * into blocks of at most min(SPICE_MAX_BLOCK, smallest projection delay) steps;
+ * per block:
+ *   1. the update callback (SPICE_PHASE_NEURONS) launches every other
+ *      population's neuron-update kernel (each applies the deliveries due,
+ *      updates, and sets spike bits);
  *   2. the runtime compacts spike bits into ordered spike lists (the delay
  *      ring) and delivers them (event counts per target);
  *   3. with plastic projections (one-step blocks), the runtime lays out each

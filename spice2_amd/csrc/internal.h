@@ -59,8 +59,9 @@ struct Pop {
 	int32_t words  = 0;
 	int32_t draws  = 0;
 	uint32_t* state = nullptr;     // [words][size]
-	uint64_t* bitmap = nullptr;    // [SPICE_MAX_BLOCK][bitmap_stride]
+	uint64_t* bitmap = nullptr;    // [bitmap rows][bitmap_stride]
 	int64_t bitmap_stride = 0;
+	bool window_bits = false;      // stateless and draws: one bitmap row per window step
 	int32_t* ring_ids = nullptr;   // [ring][size]
 	int32_t* ring_cnt = nullptr;   // [ring]
 	xoro_state* chunk_jump = nullptr; // per 16 neurons: x^(stream offset in the instance) mod P
@@ -109,6 +110,7 @@ struct PopDev {
 	int64_t inst;
 	uint64_t const* bitmap;
 	int64_t bitmap_stride;
+	int64_t window_bits;  // rows indexed by the window step (else by the block step)
 	int64_t size;
 	int32_t* ring_ids;
 	int32_t* ring_cnt;

@@ -98,7 +98,7 @@ compact_kernel(PopDev const* pops, spice_window const* win, int rel, int64_t rin
 	int64_t const t = local_step(win, P.inst, rel + k);
 	if (t < 0) return;
 	int64_t const nw = (P.size + 63) / 64;
-	uint64_t const* bm = P.bitmap + k * P.bitmap_stride;
+	uint64_t const* bm = P.bitmap + (P.window_bits ? rel + k : k) * P.bitmap_stride;
 	int64_t const slot = t % ring;
 	int32_t* out       = P.ring_ids + slot * P.size;
 
@@ -147,9 +147,10 @@ __global__ void __launch_bounds__(256) history_kernel(PopDev const* pops, spice_
 	int64_t const i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
 	if (i >= P.size) return;
 	uint64_t h = P.hist[i];
+	int64_t const row0 = P.window_bits ? rel : 0;
 	for (int u = 0; u < n; u++) {
 		if (local_step(win, P.inst, rel + u) < 0) break;
-		h = (h << 1) | ((P.bitmap[u * P.bitmap_stride + (i >> 6)] >> (i & 63)) & 1u);
+		h = (h << 1) | ((P.bitmap[(row0 + u) * P.bitmap_stride + (i >> 6)] >> (i & 63)) & 1u);
 	}
 	P.hist[i] = h;
 }
@@ -518,8 +519,8 @@ int refresh_device_descriptors(spice_ctx* ctx) {
 	std::vector<spice_pop_view> views;
 	ctx->max_pop_size = 0;
 	for (auto& P : ctx->pops) {
-		pd.push_back({P.dead ? -1 : int64_t(P.inst), P.bitmap, P.bitmap_stride, P.size, P.ring_ids, P.ring_cnt,
-		              P.hist});
+		pd.push_back({P.dead ? -1 : int64_t(P.inst), P.bitmap, P.bitmap_stride, P.window_bits ? 1 : 0, P.size,
+		              P.ring_ids, P.ring_cnt, P.hist});
 		ctx->max_pop_size = std::max(ctx->max_pop_size, P.size);
 		spice_pop_view v;
 		std::memset(&v, 0, sizeof(v));
@@ -744,6 +745,11 @@ int run_blocks(spice_ctx* ctx, int32_t nsteps, int32_t L, bool timed) {
 		if (timed) {
 			for (auto& e : tb.ev) SPICE_HIP(hipEventCreate(&e));
 			SPICE_HIP(hipEventRecord(tb.ev[0], ctx->stream));
+		}
+		if (rel == 0 && ctx->update_fn) {  // stateless populations: every step of the window
+			spice_block const blk{ctx->win_dev, ctx->views_dev, ctx->proj_views_dev, 0, nsteps, SPICE_PHASE_WINDOW};
+			if (ctx->update_fn(ctx->update_user, &blk, ctx->stream) != 0)
+				return set_error(SPICE_EINVAL, "neuron-update launch failed");
 		}
 		if (ctx->update_fn) {
 			spice_block const blk{ctx->win_dev, ctx->views_dev, ctx->proj_views_dev, rel, n, SPICE_PHASE_NEURONS};
@@ -1035,8 +1041,11 @@ int spice_pop_create_in(spice_ctx* ctx, int64_t inst, int64_t size, int32_t stat
 	P.words         = state_words;
 	P.draws         = draws;
 	P.bitmap_stride = (size + 63) / 64 + 1;  // one spare word per row: scratch for waves past the end
+	// stateless populations that draw are updated once per window (every step's spikes at once)
+	P.window_bits = state_words == 0 && draws > 0;
 	if (int r = dev_alloc(&P.state, size_t(size) * size_t(std::max(state_words, 0)))) return r;
-	if (int r = dev_alloc(&P.bitmap, size_t(SPICE_MAX_BLOCK) * P.bitmap_stride)) return r;
+	if (int r = dev_alloc(&P.bitmap, size_t(P.window_bits ? SPICE_MAX_WINDOW : SPICE_MAX_BLOCK) * P.bitmap_stride))
+		return r;
 	if (int r = dev_alloc(&P.ring_ids, size_t(ctx->ring) * size_t(size), false)) return r;
 	if (int r = dev_alloc(&P.ring_cnt, size_t(ctx->ring))) return r;
 	if (draws > 0) {
