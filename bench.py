@@ -395,9 +395,16 @@ def main():
         onet, o = run_mode(omode, args.other_steps, args.warmup, args.tile, graphs=not args.no_graphs,
                            timing_steps=min(args.timing_steps, args.other_steps), push_chunks=args.push_chunks)
         onet.close()
+        o_launches = max(o["deliver_launches"], 1)
+        o_ms = o["deliver_ms"] / o_launches
+        o_bytes = (4.0 * o["timing_events"] + 8.0 * o["timing_rows"]) / o_launches
+        o_gbs = o_bytes / (o_ms * 1e-3) / 1e9 if o_ms > 0 else None
         other = {"delivery": "pull" if args.mode == "push" else "push", "steps": args.other_steps,
                  "value": o["events"] / o["seconds"], "ms_per_step": 1e3 * o["seconds"] / args.other_steps,
                  "deliver_ms_per_step": o["deliver_ms"] / o["timing_steps"],
+                 "roofline": {"bound": "hbm", "achieved": o_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": (o_gbs / HBM_PEAK_GBS) if o_gbs else None,
+                              "algorithmic_bytes_per_launch": o_bytes, "avg_launch_ms": o_ms},
                  "note": "same network and seed; spike rasters are identical in both modes (event counts are integers)"}
     batch = None
     if not args.no_batch:
