@@ -132,6 +132,41 @@ def test_coba_network_delay_one(mode):
                 assert sh == int(z["state_hash"][t]), t
 
 
+@pytest.mark.slow
+def test_coba_benchmark_size_against_oracle():
+    """The conductance network at its benchmark size (N=250k, 80% E / 20% I,
+    p=0.004, delay 1 step; 0.27 nS and 4.5 nS over a 10 nS leak, scaled by
+    80/(N p)): spike counts of every one of the first 200 steps and the neuron
+    state every 50 steps equal the oracle's on the same seed, in both delivery
+    modes."""
+    nE, nI, p, steps = 200_000, 50_000, 0.004, 200
+    scale = 80.0 / ((nE + nI) * p)
+    wex, win = np.float32(0.027 * scale), np.float32(0.45 * scale)
+    n = O.Net(1e-4, 1e-4, [1337])
+    E = n.add(O.ORC_COBA, nE, init=1)
+    I = n.add(O.ORC_COBA, nI, init=1)
+    for s, d, w, f in ((E, E, wex, O.ORC_GEX), (E, I, wex, O.ORC_GEX), (I, E, win, O.ORC_GIN),
+                       (I, I, win, O.ORC_GIN)):
+        n.connect(s, d, p, 1e-4, w, f)
+    want, want_state = [], {}
+    for t in range(steps):
+        n.step()
+        want.append([len(n.spikes(E)), len(n.spikes(I))])
+        if t % 50 == 49:
+            want_state[t] = n.state_hash()
+    n.close()
+    assert sum(map(sum, want)) > 0
+    for mode in ("pull", "push"):
+        print("coba N=250k", mode, flush=True)
+        with S.Network.coba(nE, nI, p, float(wex), float(win), delay=1e-4, init=1, seeds=(1337,),
+                            mode=MODES[mode]) as net:
+            for t in range(steps):
+                net.step()
+                assert [len(net.spikes(0)), len(net.spikes(1))] == want[t], (mode, t)
+                if t in want_state:
+                    assert fnv(fnv(FNV, net.neurons(0)), net.neurons(1)) == want_state[t], (mode, t)
+
+
 def test_sample_program_output_is_byte_identical():
     """spice2_amd/samples/brunel.cpp (user code over the C++ API) prints the same
     JSON raster as the reference's samples/brunel.cpp."""
