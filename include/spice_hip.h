@@ -173,8 +173,8 @@ int spice_ctx_create(int device, int64_t max_delay, spice_ctx** out);
 void spice_ctx_destroy(spice_ctx* ctx);
 /* Delivery algorithm (default SPICE_DELIVER_PULL); fixed once steps have run. */
 int spice_ctx_set_delivery(spice_ctx* ctx, int mode);
-/* Spike chunks per (tile, step) of push delivery, 1..64 (0, the default: enough
- * workgroups for one per compute unit, at most 8).  Any value gives the same counts. */
+/* Spike chunks per (tile, step) of push delivery, 1..64 (0, the default: one).  Any
+ * value gives the same counts. */
 int spice_ctx_set_push_chunks(spice_ctx* ctx, int32_t chunks);
 /* Targets per delivery tile (multiple of 64, <= 32768; LDS use is 4 bytes per target and
  * projection counted together). */

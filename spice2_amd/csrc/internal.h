@@ -175,8 +175,7 @@ struct spice_ctx {
 	int64_t pull_max_tile = 64;
 	int32_t last_rel = -1, last_n = 0;  // last block run (autotuning replays its delivery)
 	int64_t tile = 8192;  // targets per pull tile
-	int32_t push_chunks = 0;  // push: spike chunks per (tile, step) (0: from the compute units)
-	int32_t cus = 256;        // compute units of the device
+	int32_t push_chunks = 0;  // push: spike chunks per (tile, step) (0: one)
 	bool dev_dirty = true;
 	int32_t* error = nullptr;                 // device flag
 	unsigned long long* events = nullptr;     // device counters: events, rows

@@ -707,22 +707,17 @@ int check_device_error(spice_ctx* ctx) {
 	return set_error(SPICE_EDEVICE, "device-side check failed (code " + std::to_string(flag) + ")");
 }
 
-// Spike chunks per (tile, step) of a push launch of n steps: the context's
-// setting, else enough workgroups for one per compute unit (at most 8 chunks).
-// Every chunk past the first adds a global atomic per counter it touches, so
-// networks that already give every compute unit a (tile, step) run unsplit.
-int32_t push_chunks(spice_ctx const* ctx, int32_t n) {
-	if (ctx->push_chunks > 0) return ctx->push_chunks;
-	int64_t const wgs   = std::max<int64_t>(ctx->pull_ntiles * n, 1);
-	int64_t const slots = int64_t(ctx->cus);
-	return int32_t(std::min<int64_t>(8, std::max<int64_t>(1, (slots + wgs - 1) / wgs)));
-}
+// Spike chunks per (tile, step) of a push launch: the context's setting, else
+// one.  Every chunk past the first adds a global atomic per counter it touches;
+// splitting was slower at every size measured (N = 20k to 500k), also where the
+// (tile, step) workgroups leave compute units idle.
+int32_t push_chunks(spice_ctx const* ctx) { return ctx->push_chunks > 0 ? ctx->push_chunks : 1; }
 
 // Delivers the spikes of steps rel .. rel+n-1 of the staged window.
 int launch_delivery(spice_ctx* ctx, int32_t rel, int32_t n) {
 	if (ctx->projs.empty() || ctx->pull_ntiles == 0) return SPICE_OK;
 	bool const push    = ctx->mode == SPICE_DELIVER_PUSH;
-	int32_t const ch   = push ? push_chunks(ctx, n) : 1;
+	int32_t const ch   = push ? push_chunks(ctx) : 1;
 	PullArgs const a{ctx->projs_dev, ctx->pull_tiles, ctx->pull_targets, ctx->win_dev, rel, n,
 	                 ctx->ring, ctx->pull_max_tile, ctx->pull_pass, ch, ctx->events};
 	dim3 const grid(unsigned(ctx->pull_ntiles * ch * SPICE_MAX_BLOCK));
@@ -828,9 +823,6 @@ int spice_ctx_create(int device, int64_t max_delay, spice_ctx** out) {
 		uint64_t keep = uint64_t(8) << 30;
 		(void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
 	}
-	int cus = 0;
-	if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
-		ctx->cus = cus;
 	if (int r = dev_alloc(&ctx->error, 1)) return r;
 	if (int r = dev_alloc(&ctx->events, 2)) return r;
 	*out = ctx;
